@@ -1,0 +1,192 @@
+#!/usr/bin/env python3
+"""Benchmark of the pfc-relax step on one MI355X.
+
+Workload: 3-component graphene/hBN heterostructure PFC on a 4096 x 4096
+periodic grid (hetero.in parameters, Gaussian coupling width ~a/2, seeded
+50/50 patchwork of ~256-point domains, noise +-0.01, conserved dynamics,
+dt = 0.25).  The same K steps run once in the mixed FP32/FP64 form (the
+headline `value`) and once in FP64; each timed region starts with the fields
+resident in HBM and is bracketed by stream synchronisation.  A "step" is one
+pass of the semi-implicit update over the grid (the reference's step(),
+src/pfc-relax.c:1133-1247, including its every-100-steps refresh).
+
+Prints one JSON line.  `roofline` is for the dominant kernel (the column
+pass), timed with HIP events on the engine's stream over the timed region;
+`cpu_baseline` times the CPU oracle on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0   # MI355X HBM3E
+
+
+def col_bytes(Nx, Ny, U, esz):
+    """Algorithmic HBM bytes of one column-pass launch: per field it reads the
+    row transforms J and the state N and writes N and the inverse-column
+    transforms I, each (Nx/2 - 1) x Ny complex (column 0 has its own kernel)."""
+    return 4 * U * (Nx // 2 - 1) * Ny * esz
+
+
+def row_bytes(Nx, Ny, U, esz):
+    """Row pass: per field reads I and writes J, (Nx/2) x Ny complex each."""
+    return 2 * U * (Nx // 2) * Ny * esz
+
+
+def survey_bytes(Nx, Ny, U, real_size):
+    """Three-sweep model of the step: U (16 NxNy + 96 (Nx/2+1) Ny) B in FP64."""
+    return U * (16 * Nx * Ny + 96 * (Nx // 2 + 1) * Ny) * real_size / 8
+
+
+def run_gpu(params, fields, Nx, Ny, dx, dy, precision, steps, warmup):
+    from pfc_amd.engine import Engine
+    g = Engine(params, Nx, Ny, dx, dy, precision)
+    g.set_fields(fields)
+    g.step(0, warmup)
+    g.nan_step()                       # synchronises the stream
+    g.timing(True)
+    t0 = time.perf_counter()
+    g.step(warmup, steps)
+    g.nan_step()
+    wall = time.perf_counter() - t0
+    ms, n = g.timing_read()
+    g.timing(False)
+    # events sit between kernels; time once more without them for ms/step
+    t0 = time.perf_counter()
+    g.step(warmup + steps, steps)
+    g.nan_step()
+    wall_plain = time.perf_counter() - t0
+    return g, wall, wall_plain, ms, n
+
+
+def cpu_baseline(params, fields, Nx, Ny, dx, dy, budget_s):
+    """Times the FP64 CPU oracle (OpenMP) on a bounded number of steps."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import Oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    o = Oracle(params, Nx, Ny, dx, dy)
+    o.set_fields(fields)
+    o.step(1)   # warm (page in)
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        o.step(2 + n)
+        n += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or n >= 50:
+            break
+    o.close()
+    return {"value": Nx * Ny * n / el, "unit": "grid-point-steps/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{n} FP64 steps of the same {Nx}x{Ny} 3-field workload on the CPU oracle "
+                      f"(oracle/pfc_oracle.c, OpenMP, {threads} threads), {el:.1f} s"}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--size", type=int, default=4096)
+    ap.add_argument("--cpu-budget", type=float, default=15.0,
+                    help="seconds of CPU-oracle stepping for the baseline (0: skip)")
+    a = ap.parse_args(argv)
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported "
+              "(the whole grid lives on one MI355X; see the sharding notes)", file=sys.stderr)
+        return 2
+
+    from pfc_amd import FP64, MIXED, LATTICE_A, hetero_params
+    from pfc_amd.inputs import hetero_patchwork
+
+    Nx = Ny = a.size
+    U = 3
+    dx = dy = 0.7
+    sigma = 2.0 / 7.2552          # real-space Gaussian width ~ a/2
+    params = hetero_params(dt=0.25, c=1.0, sigma=sigma)
+    t0 = time.perf_counter()
+    fields = hetero_patchwork(Nx, Ny, dx, dy, domain=256, seed=2024, noise=0.01)
+    gen_s = time.perf_counter() - t0
+
+    res = {}
+    for name, prec in (("mixed", MIXED), ("fp64", FP64)):
+        g, wall, wall_plain, ms, n = run_gpu(params, fields, Nx, Ny, dx, dy, prec, a.steps, a.warmup)
+        esz = 16 if prec == FP64 else 8
+        col_ms = ms[2] / max(n, 1)
+        row_ms = ms[0] / max(n, 1)
+        res[name] = dict(
+            wall=wall_plain, wall_evented=wall, ms_step=1e3 * wall_plain / a.steps,
+            gps=Nx * Ny * a.steps / wall_plain,
+            col_ms=col_ms, row_ms=row_ms, col0_ms=ms[1] / max(n, 1),
+            col_gbs=col_bytes(Nx, Ny, U, esz) / (col_ms * 1e-3) / 1e9,
+            row_gbs=row_bytes(Nx, Ny, U, esz) / (row_ms * 1e-3) / 1e9,
+            step_model_gbs=survey_bytes(Nx, Ny, U, esz // 2) * a.steps / wall_plain / 1e9,
+        )
+        fe, ne = g.evaluate()
+        res[name]["fave"] = fe
+        res[name]["fields"] = g.get_fields()
+        g.close()
+
+    m, d = res["mixed"], res["fp64"]
+    err_l2 = float(np.linalg.norm(m["fields"] - d["fields"]) / np.linalg.norm(d["fields"]))
+    err_f = abs(m["fave"] - d["fave"]) / abs(d["fave"])
+
+    cpu = None
+    if a.cpu_budget > 0:
+        cpu = cpu_baseline(params, fields, Nx, Ny, dx, dy, a.cpu_budget)
+
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic_col_step_4096x4096_u3.txt")
+    if os.path.exists(tf):
+        for line in open(tf):
+            if line.startswith("mixed_bytes_per_launch:"):
+                traffic = float(line.split(":")[1])
+
+    out = {
+        "metric": "grid-point-steps/s at 4096² 3-field, mixed vs FP64 on 1 GPU; HBM GB/s",
+        "value": m["gps"],
+        "unit": "grid-point-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": m["ms_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (f64 reductions, k=0 mode, box)",
+        "data": "synthetic: seeded graphene/hBN Voronoi patchwork (hetero.in parameters)",
+        "config": {"workload": f"hetero 3-field PFC {Nx}x{Ny} conserved dt=0.25, mixed precision",
+                   "grid": [Nx, Ny], "fields": U, "parallelism": "1 GPU"},
+        "fp64": {"value": d["gps"], "ms_per_step": d["ms_step"],
+                 "column_pass_GBps": d["col_gbs"], "row_pass_GBps": d["row_gbs"],
+                 "column_pass_ms": d["col_ms"], "row_pass_ms": d["row_ms"],
+                 "step_model_GBps": d["step_model_gbs"]},
+        "mixed": {"column_pass_GBps": m["col_gbs"], "row_pass_GBps": m["row_gbs"],
+                  "column_pass_ms": m["col_ms"], "row_pass_ms": m["row_ms"],
+                  "column0_pass_ms": m["col0_ms"], "step_model_GBps": m["step_model_gbs"]},
+        "speedup_mixed_vs_fp64": m["gps"] / d["gps"],
+        "mixed_vs_fp64_error": {"density_rel_l2": err_l2, "energy_per_area_rel": err_f,
+                                "after_steps": a.warmup + 2 * a.steps},
+        "roofline": {"bound": "hbm", "achieved": m["col_gbs"], "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": m["col_gbs"] / PEAK_HBM_GBS, "traffic": traffic,
+                     "kernel": "k_col_step (column pass, mixed)"},
+        "cpu_baseline": cpu,
+        "input_generation_s": gen_s,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,132 @@
+/*
+ * pfc_relax.h -- C ABI of libpfcgpu, the MI355X (gfx950) engine behind the
+ * pfc-relax drop-in binary.
+ *
+ * The reference (petenez/pfc, src/pfc-relax.c) is one C translation unit with
+ * no library API: its "boundary" is the executable plus text files.  The
+ * functions below are the seam a maintainer would bind to replace the solver
+ * core of that file; each one names the reference function(s) it stands in
+ * for.  All arguments are plain C types and host pointers; device memory is
+ * private to the context.  Every call returns 0 on success or a negative
+ * PFC_E* code and never exits the process.
+ *
+ * A context is not thread safe: use one per host thread.
+ */
+#ifndef PFC_RELAX_H
+#define PFC_RELAX_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PFC_MAX_FIELDS 8   /* largest U accepted */
+#define PFC_MAX_MODES 8    /* largest W[u] accepted */
+
+/* Model parameters, as read from the "M" and "R" blocks of an input file
+ * (reference: struct S, src/pfc-relax.c:34-120; parsed at :541-718).
+ * Matrices are indexed [u][v] and only the upper triangle is read:
+ * alpha/beta/gamma use v >= u, epsilon/mu2 use v > u.
+ * nu2 and mu2 hold 1/nu^2 and 1/mu^2 (the file stores nu and mu, :654, :699);
+ * mu2[u][v] is 0 when beta[u][v] == 0 (:698). */
+typedef struct pfc_params {
+    int U;                                        /* number of density fields */
+    int W[PFC_MAX_FIELDS];                        /* modes per field */
+    int energy_compat;                            /* see PFC_ENERGY_* below */
+    int reserved;
+    double c;                                     /* 0: nonconserved, else conserved */
+    double sigma;                                 /* Gaussian width in k space */
+    double dt;                                    /* time step */
+    double lambda[PFC_MAX_FIELDS];
+    double delta[PFC_MAX_FIELDS];
+    double alpha[PFC_MAX_FIELDS][PFC_MAX_FIELDS];
+    double beta[PFC_MAX_FIELDS][PFC_MAX_FIELDS];
+    double gamma[PFC_MAX_FIELDS][PFC_MAX_FIELDS];
+    double epsilon[PFC_MAX_FIELDS][PFC_MAX_FIELDS];
+    double mu2[PFC_MAX_FIELDS][PFC_MAX_FIELDS];
+    double nu2[PFC_MAX_FIELDS][PFC_MAX_MODES];
+    double omega[PFC_MAX_FIELDS][PFC_MAX_MODES];
+    double phi[PFC_MAX_FIELDS][PFC_MAX_MODES];
+} pfc_params;
+
+/* energy_compat: which coefficient the free energy's beta_uv gradient
+ * coupling uses.  The reference's evaluate() aliases mu2 to nu2
+ * (src/pfc-relax.c:1019) and then reads nu2[u][v] (:1064), past the end of
+ * nu2[u] when v >= W[u].  PFC_ENERGY_MODEL evaluates the documented model
+ * (mu2, :9); PFC_ENERGY_NU2 reproduces the alias, using nu2[u][v] when
+ * v < W[u] and 0 otherwise.  Dynamics are unaffected (step() uses mu2). */
+#define PFC_ENERGY_MODEL 0
+#define PFC_ENERGY_NU2 1
+
+/* precision */
+#define PFC_FP64 0    /* all-FP64 FFTs, update and reductions */
+#define PFC_MIXED 1   /* FP32 FFTs/update, FP64 reductions, k=0 mode and box */
+
+/* error codes */
+#define PFC_OK 0
+#define PFC_EINVAL (-1)     /* bad argument (size, U, W, pointer) */
+#define PFC_EDEVICE (-2)    /* HIP runtime error */
+#define PFC_ENOMEM (-3)     /* device allocation failed */
+#define PFC_ESIZE (-4)      /* grid size has no kernel (Nx, Ny must be 2^k in [64, 8192]) */
+#define PFC_ESTATE (-5)     /* call out of order (e.g. step before set_fields) */
+
+typedef struct pfc_ctx pfc_ctx;
+
+/* Allocates device state for an Nx x Ny grid of p->U fields and compiles no
+ * plan: kernels and twiddle tables are fixed per size.
+ * Replaces allocate_parameters/allocate_data/plan_FFTs (src/pfc-relax.c:123-239). */
+int pfc_create(pfc_ctx** ctx, const pfc_params* p, int Nx, int Ny, int precision);
+
+/* Uploads U density fields, each Ny rows of Nx values (row-major, x fastest,
+ * the order of the density files), and synchronises the spectral state to
+ * them (N = FFT(n)/NxNy).  Replaces read_data's store into m (:728-763). */
+int pfc_set_fields(pfc_ctx* ctx, const double* fields);
+
+/* Sets the discretisation; A and B operators follow from it.
+ * Replaces the dx/dy assignment plus update_AB (:900-957). */
+int pfc_set_box(pfc_ctx* ctx, double dx, double dy);
+
+/* Runs nsteps semi-implicit steps t0, t0+1, ...; for each step t with
+ * t % 100 == 0 the spectral state is first re-synchronised to the real-space
+ * field.  A NaN in field 0 at grid point (0,0) at the start of a step is
+ * recorded (see pfc_nan_step) and the remaining steps still run.
+ * Replaces step() (:1133-1247). */
+int pfc_step(pfc_ctx* ctx, int t0, int nsteps);
+
+/* Free energy per unit area and mean densities (nave has U entries); the
+ * energy density field is kept on the device (pfc_get_energy) and the
+ * spectral state is re-synchronised, as in the reference.
+ * Replaces evaluate() incl. its MPI_Allreduce (:987-1130). */
+int pfc_evaluate(pfc_ctx* ctx, double* fave, double* nave);
+
+/* Downloads the current density fields (layout as pfc_set_fields).
+ * Feeds write_data (:766-813). */
+int pfc_get_fields(pfc_ctx* ctx, double* fields);
+
+/* Downloads the energy density of the last pfc_evaluate, Ny x Nx values
+ * (zeros before the first evaluate).  Feeds write_energy (:816-858). */
+int pfc_get_energy(pfc_ctx* ctx, double* f);
+
+/* First step index whose start saw a NaN at field 0, point (0,0), or -1.
+ * Replaces the check at :1175 (fail code 39). */
+int pfc_nan_step(pfc_ctx* ctx, int* step);
+
+/* Measurement hooks; no counterpart in the reference (it only prints whole
+ * seconds, :880).  While enabled, pfc_step brackets each of its passes with
+ * HIP events on the context's stream.  pfc_timing(ctx, 1) also clears what
+ * was collected.  pfc_timing_read waits for the stream and returns the summed
+ * milliseconds of the row pass (ms[0]), the packed-column-0 pass (ms[1]) and
+ * the column pass (ms[2]) over *nsteps recorded steps. */
+int pfc_timing(pfc_ctx* ctx, int enable);
+int pfc_timing_read(pfc_ctx* ctx, double* ms, int* nsteps);
+
+/* Releases everything.  Replaces clean() (:1315-1389). */
+int pfc_destroy(pfc_ctx* ctx);
+
+/* Human-readable text for a PFC_E* code. */
+const char* pfc_strerror(int code);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

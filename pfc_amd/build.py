@@ -1,0 +1,89 @@
+"""Builds libpfcgpu.so (HIP, gfx950) and the pfc-relax host binary in-tree.
+
+    python -m pfc_amd.build            # everything
+    python -m pfc_amd.build --lib      # only the HIP library
+
+Outputs: pfc_amd/lib/libpfcgpu.so and pfc_amd/bin/pfc-relax (rpath to ../lib).
+The three HIP translation units (FP64 kernels, FP32 kernels, engine) compile
+in parallel.
+"""
+from __future__ import annotations
+
+import argparse
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+HOST = os.path.join(PKG, "host")
+LIBDIR = os.path.join(PKG, "lib")
+BINDIR = os.path.join(PKG, "bin")
+OBJDIR = os.path.join(ROOT, "_build")
+
+HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("PFC_OFFLOAD_ARCH", "gfx950")
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
+    # no packed FP32 instructions (v_pk_*_f32) anywhere in the device code
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-pass-failed",
+]
+HIP_UNITS = ["pfc_launch_f64.hip", "pfc_launch_f32.hip", "pfc_engine.hip"]
+HOST_SRCS = ["pfc_relax_main.c", "pfc_io.c"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l)
+    if r.returncode != 0:
+        raise RuntimeError(f"command failed ({r.returncode}): {' '.join(cmd)}\n{out}")
+    return out
+
+
+def build_lib(verbose=False):
+    os.makedirs(OBJDIR, exist_ok=True)
+    os.makedirs(LIBDIR, exist_ok=True)
+    objs = []
+    cmds = []
+    for u in HIP_UNITS:
+        obj = os.path.join(OBJDIR, u.replace(".hip", ".o"))
+        objs.append(obj)
+        cmds.append([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, u), "-o", obj])
+    with cf.ThreadPoolExecutor(max_workers=len(cmds)) as ex:
+        for out in ex.map(_run, cmds):
+            if verbose and out.strip():
+                print(out)
+    lib = os.path.join(LIBDIR, "libpfcgpu.so")
+    tmp = lib + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    os.replace(tmp, lib)
+    return lib
+
+
+def build_host():
+    os.makedirs(BINDIR, exist_ok=True)
+    exe = os.path.join(BINDIR, "pfc-relax")
+    srcs = [os.path.join(HOST, s) for s in HOST_SRCS]
+    _run(["gcc", "-O2", "-std=gnu99", "-Wall", "-Wextra", "-pthread", "-I", os.path.join(ROOT, "include"),
+          *srcs, "-o", exe + ".tmp", "-L", LIBDIR, "-lpfcgpu", "-Wl,-rpath,$ORIGIN/../lib", "-lm"])
+    os.replace(exe + ".tmp", exe)
+    return exe
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--lib", action="store_true", help="only the HIP library")
+    ap.add_argument("-v", "--verbose", action="store_true")
+    a = ap.parse_args(argv)
+    lib = build_lib(a.verbose)
+    print("built", lib)
+    if not a.lib:
+        print("built", build_host())
+
+
+if __name__ == "__main__":
+    sys.exit(main())

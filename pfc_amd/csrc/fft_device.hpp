@@ -1,0 +1,187 @@
+// fft_device.hpp -- register/LDS Stockham FFT building blocks for gfx950.
+//
+// A length-L transform is computed by TPB threads, each holding P = L/TPB
+// points in registers.  On entry and on exit thread t holds elements
+// t + q*TPB (q < P) in natural order, so the first and last stages of every
+// transform read/write global memory with unit stride across a wavefront and
+// only the L/P-strided middle stages go through LDS.
+//
+// Stage plan: one leading stage of radix R0 = L / P^s (when L is not a power
+// of P, done in registers with P/R0 butterflies per thread), then s stages of
+// radix P (Stockham autosort).  Twiddles come from a table tw[m] =
+// exp(-2 pi i m / (L*TWS)) computed on the host in extended precision; the
+// inverse uses the conjugate.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace pfc {
+
+template <typename T>
+struct cpx {
+    T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ cpx<T> cadd(cpx<T> a, cpx<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T>
+__device__ __forceinline__ cpx<T> csub(cpx<T> a, cpx<T> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename T>
+__device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b) {
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+// a * conj(b)
+template <typename T>
+__device__ __forceinline__ cpx<T> cmulc(cpx<T> a, cpx<T> b) {
+    return {a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y};
+}
+template <typename T>
+__device__ __forceinline__ cpx<T> cconj(cpx<T> a) { return {a.x, -a.y}; }
+template <typename T>
+__device__ __forceinline__ cpx<T> cscale(cpx<T> a, T s) { return {a.x * s, a.y * s}; }
+// a * (DIR * i)
+template <int DIR, typename T>
+__device__ __forceinline__ cpx<T> cmul_i(cpx<T> a) {
+    if constexpr (DIR > 0) return {-a.y, a.x};
+    else return {a.y, -a.x};
+}
+
+// cos/sin(2 pi k / 16), k = 0..15 (folded at compile time)
+__host__ __device__ constexpr double c16(int k) {
+    constexpr double a = 0.92387953251128675613, b = 0.70710678118654752440,
+                     c = 0.38268343236508977173;
+    switch (k & 15) {
+        case 0: return 1.0;  case 1: return a;   case 2: return b;   case 3: return c;
+        case 4: return 0.0;  case 5: return -c;  case 6: return -b;  case 7: return -a;
+        case 8: return -1.0; case 9: return -a;  case 10: return -b; case 11: return -c;
+        case 12: return 0.0; case 13: return c;  case 14: return b;  default: return a;
+    }
+}
+__host__ __device__ constexpr double s16(int k) { return c16(k - 4); }
+
+// In-register DFT of size R (1, 2, 4, 8, 16): v[k] <- sum_r v[r] e^{DIR 2 pi i rk/R}
+template <int R, int DIR, typename T>
+__device__ __forceinline__ void dft(cpx<T>* v) {
+    if constexpr (R == 1) {
+        return;
+    } else if constexpr (R == 2) {
+        cpx<T> a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    } else {
+        constexpr int H = R / 2;
+        cpx<T> e[H], o[H];
+#pragma unroll
+        for (int i = 0; i < H; i++) {
+            e[i] = v[2 * i];
+            o[i] = v[2 * i + 1];
+        }
+        dft<H, DIR, T>(e);
+        dft<H, DIR, T>(o);
+#pragma unroll
+        for (int k = 0; k < H; k++) {
+            cpx<T> t;
+            if (k == 0) {
+                t = o[k];
+            } else if (4 * k == R) {
+                t = cmul_i<DIR>(o[k]);
+            } else {
+                constexpr int step = 16 / R;
+                const cpx<T> w = {(T)c16(k * step), (T)(DIR * s16(k * step))};
+                t = cmul(o[k], w);
+            }
+            v[k] = cadd(e[k], t);
+            v[k + H] = csub(e[k], t);
+        }
+    }
+}
+
+constexpr int ilog(int b, int n) { return n < b ? 0 : 1 + ilog(b, n / b); }
+constexpr int ipow(int b, int e) { return e == 0 ? 1 : b * ipow(b, e - 1); }
+
+// One length-L transform over TPB cooperating threads (thread index t).
+// lds must hold L elements at lds[idx * LS] (LS = element stride, lets
+// several interleaved transforms share a buffer without bank conflicts).
+template <typename T, int L, int TPB, int DIR, int TWS, int LS>
+struct BlockFFT {
+    static constexpr int P = L / TPB;
+    static_assert(P * TPB == L, "L must be a multiple of TPB");
+    static_assert(P == 2 || P == 4 || P == 8 || P == 16, "P must be 2, 4, 8 or 16");
+    static constexpr int S = ilog(P, L);          // number of radix-P stages
+    static constexpr int R0 = L / ipow(P, S);     // leading stage radix (1 if none)
+
+    __device__ static __forceinline__ cpx<T> twid(const cpx<T>* __restrict__ tw, int m) {
+        cpx<T> w = tw[m * TWS];
+        if constexpr (DIR > 0) w.y = -w.y;
+        return w;
+    }
+
+    // One radix-R Stockham stage on butterflies j = t + b*TPB, inputs in in[].
+    template <int R, bool LAST>
+    __device__ static __forceinline__ void stage(cpx<T> (&in)[P], int Ns, cpx<T>* lds,
+                                                 const cpx<T>* __restrict__ tw, int t) {
+        constexpr int B = P / R;
+#pragma unroll
+        for (int b = 0; b < B; b++) {
+            const int j = t + b * TPB;
+            const int k = j & (Ns - 1);
+            cpx<T> x[R];
+#pragma unroll
+            for (int r = 0; r < R; r++) x[r] = in[b + r * B];
+            if (Ns > 1) {
+                const int step = L / (Ns * R);
+#pragma unroll
+                for (int r = 1; r < R; r++) x[r] = cmul(x[r], twid(tw, k * r * step));
+            }
+            dft<R, DIR, T>(x);
+            if constexpr (LAST) {
+#pragma unroll
+                for (int r = 0; r < R; r++) in[b + r * B] = x[r];
+            } else {
+                const int base = (j - k) * R + k;
+#pragma unroll
+                for (int r = 0; r < R; r++) lds[(base + r * Ns) * LS] = x[r];
+            }
+        }
+    }
+
+    // Gather the inputs of a radix-R stage from LDS.
+    template <int R>
+    __device__ static __forceinline__ void gather(cpx<T> (&in)[P], const cpx<T>* lds, int t) {
+        constexpr int B = P / R;
+#pragma unroll
+        for (int b = 0; b < B; b++) {
+            const int j = t + b * TPB;
+#pragma unroll
+            for (int r = 0; r < R; r++) in[b + r * B] = lds[(j + r * (L / R)) * LS];
+        }
+    }
+
+    // v: natural-order input (thread t holds t + q*TPB); same layout on exit.
+    // Begins with a barrier so lds may still be in use by the caller.
+    __device__ static __forceinline__ void run(cpx<T> (&v)[P], cpx<T>* lds,
+                                               const cpx<T>* __restrict__ tw, int t) {
+        int Ns = 1;
+        __syncthreads();
+        if constexpr (R0 > 1) {
+            // in-register leading stage: butterfly b of thread t uses v[b + r*(P/R0)]
+            stage<R0, false>(v, 1, lds, tw, t);
+            Ns = R0;
+            __syncthreads();
+            gather<P>(v, lds, t);
+        }
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+            if (s == S - 1) {
+                stage<P, true>(v, Ns, lds, tw, t);
+            } else {
+                if (s > 0 || R0 > 1) __syncthreads();
+                stage<P, false>(v, Ns, lds, tw, t);
+                Ns *= P;
+                __syncthreads();
+                gather<P>(v, lds, t);
+            }
+        }
+    }
+};
+
+}  // namespace pfc

@@ -1,0 +1,380 @@
+// pfc_engine.hip -- libpfcgpu: the C ABI of include/pfc_relax.h on gfx950.
+//
+// Device state of a context (E = complex element, FP64 or FP32 by precision;
+// nk = (Nx/2)*Ny, nr = Nx*Ny):
+//   I, J, K, Nh   U*nk E each  (layout in pfc_kernels.hpp)
+//   O             (2U + U(U-1)/2)*nk E   evaluate's k-space outputs
+//   mreal         U*nr real    densities in real space (materialised on demand)
+//   R             (2U + U(U-1)/2)*nr real  evaluate's real-space outputs
+//   f             nr doubles   energy density of the last evaluate
+// Everything runs on one HIP stream owned by the context.
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "../../include/pfc_relax.h"
+#include "pfc_launch.hpp"
+
+extern template struct pfc::Launch<double>;
+extern template struct pfc::Launch<float>;
+
+using namespace pfc;
+
+namespace {
+
+using pfc::Launch;
+
+#define HIPCHK(x)                                 \
+    do {                                          \
+        hipError_t e_ = (x);                      \
+        if (e_ != hipSuccess) return PFC_EDEVICE; \
+    } while (0)
+
+template <typename T>
+void fill_twiddles(std::vector<cpx<T>>& tw, int n) {
+    tw.resize(n);
+    const long double pi = 3.141592653589793238462643383279502884L;
+    for (int m = 0; m < n; m++) {
+        const long double a = -2.0L * pi * (long double)m / (long double)n;
+        tw[m].x = (T)cosl(a);
+        tw[m].y = (T)sinl(a);
+    }
+}
+
+void fill_devparams(pfc_ctx* c) {
+    DevParams& d = c->hp;
+    const pfc_params& p = c->p;
+    std::memset(&d, 0, sizeof(d));
+    d.U = p.U;
+    for (int u = 0; u < p.U; u++) d.W[u] = p.W[u];
+    d.energy_compat = p.energy_compat;
+    d.c1 = (p.c != 0.0) ? 1.0 : 0.0;
+    d.dt = p.dt;
+    d.sig_a = -0.5 / p.sigma / p.sigma;
+    d.NxNy = (double)c->Nx * c->Ny;
+    d.inv_NxNy = 1.0 / c->Nx / c->Ny;
+    const double pi = 3.14159265358979323846264338327;
+    d.dkx = 2.0 * pi / (c->dx * c->Nx);
+    d.dky = 2.0 * pi / (c->dy * c->Ny);
+    for (int u = 0; u < p.U; u++) {
+        d.lambda[u] = p.lambda[u];
+        d.delta[u] = p.delta[u];
+        for (int v = 0; v < p.U; v++) {
+            d.alpha[u][v] = p.alpha[u][v];
+            d.beta[u][v] = p.beta[u][v];
+            d.gamma[u][v] = p.gamma[u][v];
+            d.epsilon[u][v] = p.epsilon[u][v];
+            d.mu2[u][v] = p.mu2[u][v];
+            if (p.energy_compat == PFC_ENERGY_NU2)
+                d.qc2[u][v] = (v < p.W[u]) ? p.nu2[u][v] : 0.0;
+            else
+                d.qc2[u][v] = p.mu2[u][v];
+        }
+        for (int w = 0; w < p.W[u]; w++) {
+            d.nu2[u][w] = p.nu2[u][w];
+            d.omega[u][w] = p.omega[u][w];
+            d.phi[u][w] = p.phi[u][w];
+        }
+    }
+}
+
+int upload_params(pfc_ctx* c) {
+    fill_devparams(c);
+    HIPCHK(hipMemcpyAsync(c->dprm, &c->hp, sizeof(DevParams), hipMemcpyHostToDevice, c->stream));
+    return PFC_OK;
+}
+
+void free_all(pfc_ctx* c) {
+    void* bufs[] = {c->I, c->J, c->K, c->Nh, c->O, c->mreal, c->R, c->twx, c->twy,
+                    c->f, c->partials, c->sums, c->nan_flag, c->dprm};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    for (int i = 0; i < c->nev; i++) (void)hipEventDestroy(c->ev[i]);
+    free(c->ev);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+// Grows the event pool to hold n recorded steps (4 events each).
+int ensure_events(pfc_ctx* c, int n) {
+    if (4 * n <= c->nev) return PFC_OK;
+    int cap = c->nev ? c->nev : 64;
+    while (cap < 4 * n) cap *= 2;
+    hipEvent_t* ev = (hipEvent_t*)realloc(c->ev, sizeof(hipEvent_t) * cap);
+    if (!ev) return PFC_ENOMEM;
+    for (int i = c->nev; i < cap; i++)
+        if (hipEventCreate(&ev[i]) != hipSuccess) return PFC_EDEVICE;
+    c->ev = ev;
+    c->nev = cap;
+    return PFC_OK;
+}
+
+int dalloc(void** p, size_t bytes) {
+    if (hipMalloc(p, bytes) != hipSuccess) {
+        *p = nullptr;
+        return PFC_ENOMEM;
+    }
+    return PFC_OK;
+}
+
+// Materialise the densities in mreal; if K, also refresh Nh from them.
+int materialise(pfc_ctx* c, bool refresh) {
+    if (c->prec == PFC_FP64) {
+        Launch<double>::row_c2r_any(c, c->I, c->mreal, refresh ? c->K : nullptr, c->U);
+        if (refresh) Launch<double>::col_fwd_any(c, c->K, c->Nh, c->U, c->hp.inv_NxNy);
+    } else {
+        Launch<float>::row_c2r_any(c, c->I, c->mreal, refresh ? c->K : nullptr, c->U);
+        if (refresh) Launch<float>::col_fwd_any(c, c->K, c->Nh, c->U, c->hp.inv_NxNy);
+    }
+    HIPCHK(hipGetLastError());
+    return PFC_OK;
+}
+
+}  // namespace
+
+// ============================================================== C ABI
+
+extern "C" {
+
+const char* pfc_strerror(int code) {
+    switch (code) {
+        case PFC_OK: return "success";
+        case PFC_EINVAL: return "invalid argument";
+        case PFC_EDEVICE: return "HIP runtime error";
+        case PFC_ENOMEM: return "device allocation failed";
+        case PFC_ESIZE: return "unsupported grid size (Nx and Ny must be powers of two in [64, 8192])";
+        case PFC_ESTATE: return "call out of order";
+        default: return "unknown error";
+    }
+}
+
+int pfc_create(pfc_ctx** out, const pfc_params* p, int Nx, int Ny, int precision) {
+    if (!out || !p) return PFC_EINVAL;
+    *out = nullptr;
+    if (p->U < 1 || p->U > kMaxU) return PFC_EINVAL;
+    for (int u = 0; u < p->U; u++)
+        if (p->W[u] < 0 || p->W[u] > PFC_MAX_MODES) return PFC_EINVAL;
+    if (precision != PFC_FP64 && precision != PFC_MIXED) return PFC_EINVAL;
+    if (!size_ok(Nx) || !size_ok(Ny)) return PFC_ESIZE;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return PFC_EDEVICE;
+
+    pfc_ctx* c = (pfc_ctx*)calloc(1, sizeof(pfc_ctx));
+    if (!c) return PFC_ENOMEM;
+    c->Nx = Nx;
+    c->Ny = Ny;
+    c->U = p->U;
+    c->prec = precision;
+    c->p = *p;
+    c->nO = 2 * c->U + c->U * (c->U - 1) / 2;
+    c->nk = (size_t)(Nx / 2) * Ny;
+    c->nr = (size_t)Nx * Ny;
+    c->esz = (precision == PFC_FP64) ? 16 : 8;
+    const size_t rsz = c->esz / 2;
+    c->dx = 1.0;
+    c->dy = 1.0;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        free(c);
+        return PFC_EDEVICE;
+    }
+    int rc = PFC_OK;
+    const size_t spec = c->U * c->nk * c->esz;
+    rc |= dalloc(&c->I, spec);
+    rc |= dalloc(&c->J, spec);
+    rc |= dalloc(&c->K, spec);
+    rc |= dalloc(&c->Nh, spec);
+    rc |= dalloc(&c->O, c->nO * c->nk * c->esz);
+    rc |= dalloc(&c->mreal, c->U * c->nr * rsz);
+    rc |= dalloc(&c->R, c->nO * c->nr * rsz);
+    rc |= dalloc((void**)&c->f, c->nr * sizeof(double));
+    rc |= dalloc((void**)&c->partials, (size_t)kEnergyBlocks * 2 * (1 + c->U) * sizeof(double));
+    rc |= dalloc((void**)&c->sums, (1 + c->U) * sizeof(double));
+    rc |= dalloc((void**)&c->nan_flag, sizeof(int));
+    rc |= dalloc((void**)&c->dprm, sizeof(DevParams));
+    rc |= dalloc(&c->twx, Nx * c->esz);
+    rc |= dalloc(&c->twy, Ny * c->esz);
+    if (rc != PFC_OK) {
+        free_all(c);
+        free(c);
+        return PFC_ENOMEM;
+    }
+    bool ok = true;
+    if (precision == PFC_FP64) {
+        std::vector<cpx<double>> tx, ty;
+        fill_twiddles(tx, Nx);
+        fill_twiddles(ty, Ny);
+        ok &= hipMemcpy(c->twx, tx.data(), Nx * c->esz, hipMemcpyHostToDevice) == hipSuccess;
+        ok &= hipMemcpy(c->twy, ty.data(), Ny * c->esz, hipMemcpyHostToDevice) == hipSuccess;
+    } else {
+        std::vector<cpx<float>> tx, ty;
+        fill_twiddles(tx, Nx);
+        fill_twiddles(ty, Ny);
+        ok &= hipMemcpy(c->twx, tx.data(), Nx * c->esz, hipMemcpyHostToDevice) == hipSuccess;
+        ok &= hipMemcpy(c->twy, ty.data(), Ny * c->esz, hipMemcpyHostToDevice) == hipSuccess;
+    }
+    const int big = INT_MAX;
+    ok &= hipMemcpy(c->nan_flag, &big, sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
+    ok &= hipMemset(c->f, 0, c->nr * sizeof(double)) == hipSuccess;
+    if (!ok || upload_params(c) != PFC_OK || hipStreamSynchronize(c->stream) != hipSuccess) {
+        free_all(c);
+        free(c);
+        return PFC_EDEVICE;
+    }
+    *out = c;
+    return PFC_OK;
+}
+
+int pfc_destroy(pfc_ctx* c) {
+    if (!c) return PFC_EINVAL;
+    (void)hipStreamSynchronize(c->stream);
+    free_all(c);
+    free(c);
+    return PFC_OK;
+}
+
+int pfc_set_box(pfc_ctx* c, double dx, double dy) {
+    if (!c) return PFC_EINVAL;
+    c->dx = dx;
+    c->dy = dy;
+    c->have_box = 1;
+    return upload_params(c);
+}
+
+int pfc_set_fields(pfc_ctx* c, const double* fields) {
+    if (!c || !fields) return PFC_EINVAL;
+    const size_t n = c->U * c->nr;
+    if (c->prec == PFC_FP64) {
+        HIPCHK(hipMemcpyAsync(c->mreal, fields, n * sizeof(double), hipMemcpyHostToDevice,
+                              c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        Launch<double>::row_r2c_any(c, c->mreal, c->I, c->U);
+        Launch<double>::col_fwd_any(c, c->I, c->Nh, c->U, 1.0 / c->Ny);  // I carries 1/Nx
+    } else {
+        std::vector<float> tmp(n);
+        for (size_t i = 0; i < n; i++) tmp[i] = (float)fields[i];
+        HIPCHK(hipMemcpyAsync(c->mreal, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice,
+                              c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        Launch<float>::row_r2c_any(c, c->mreal, c->I, c->U);
+        Launch<float>::col_fwd_any(c, c->I, c->Nh, c->U, 1.0 / c->Ny);  // I carries 1/Nx
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->have_fields = 1;
+    return PFC_OK;
+}
+
+int pfc_step(pfc_ctx* c, int t0, int nsteps) {
+    if (!c || nsteps < 0) return PFC_EINVAL;
+    if (!c->have_fields || !c->have_box) return PFC_ESTATE;
+    for (int i = 0; i < nsteps; i++) {
+        const int t = t0 + i;
+        const bool ref = (t % 100 == 0);
+        hipEvent_t* ev = nullptr;
+        if (c->timing) {
+            if (ensure_events(c, c->nrec + 1) != PFC_OK) return PFC_EDEVICE;
+            ev = c->ev + 4 * c->nrec++;
+            HIPCHK(hipEventRecord(ev[0], c->stream));
+        }
+        if (c->prec == PFC_FP64) Launch<double>::row_step_any(c, t, ref);
+        else Launch<float>::row_step_any(c, t, ref);
+        if (ev) HIPCHK(hipEventRecord(ev[1], c->stream));
+        if (c->prec == PFC_FP64) Launch<double>::col_step_any(c, ref, ev ? ev[2] : nullptr);
+        else Launch<float>::col_step_any(c, ref, ev ? ev[2] : nullptr);
+        if (ev) HIPCHK(hipEventRecord(ev[3], c->stream));
+    }
+    HIPCHK(hipGetLastError());
+    return PFC_OK;
+}
+
+int pfc_timing(pfc_ctx* c, int enable) {
+    if (!c) return PFC_EINVAL;
+    c->timing = enable ? 1 : 0;
+    if (enable) c->nrec = 0;
+    return PFC_OK;
+}
+
+int pfc_timing_read(pfc_ctx* c, double* ms, int* nsteps) {
+    if (!c || !ms || !nsteps) return PFC_EINVAL;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    ms[0] = ms[1] = ms[2] = 0.0;
+    for (int i = 0; i < c->nrec; i++) {
+        hipEvent_t* e = c->ev + 4 * i;
+        for (int k = 0; k < 3; k++) {
+            float v = 0.f;
+            HIPCHK(hipEventElapsedTime(&v, e[k], e[k + 1]));
+            ms[k] += v;
+        }
+    }
+    *nsteps = c->nrec;
+    return PFC_OK;
+}
+
+int pfc_evaluate(pfc_ctx* c, double* fave, double* nave) {
+    if (!c || !fave || !nave) return PFC_EINVAL;
+    if (!c->have_fields || !c->have_box) return PFC_ESTATE;
+    if (c->prec == PFC_FP64) {
+        Launch<double>::row_c2r_any(c, c->I, c->mreal, c->K, c->U);
+        Launch<double>::col_eval_any(c);
+        Launch<double>::row_c2r_any(c, c->O, c->R, nullptr, c->nO);
+        Launch<double>::energy_any(c);
+    } else {
+        Launch<float>::row_c2r_any(c, c->I, c->mreal, c->K, c->U);
+        Launch<float>::col_eval_any(c);
+        Launch<float>::row_c2r_any(c, c->O, c->R, nullptr, c->nO);
+        Launch<float>::energy_any(c);
+    }
+    hipLaunchKernelGGL((k_reduce_final<1024>), dim3(1), dim3(1024), 0, c->stream, c->partials,
+                       kEnergyBlocks, 1 + c->U, c->sums, c->hp.inv_NxNy);
+    HIPCHK(hipGetLastError());
+    double h[1 + PFC_MAX_FIELDS];
+    HIPCHK(hipMemcpyAsync(h, c->sums, (1 + c->U) * sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    *fave = h[0];
+    for (int u = 0; u < c->U; u++) nave[u] = h[1 + u];
+    c->have_f = 1;
+    return PFC_OK;
+}
+
+int pfc_get_fields(pfc_ctx* c, double* fields) {
+    if (!c || !fields) return PFC_EINVAL;
+    if (!c->have_fields) return PFC_ESTATE;
+    int rc = materialise(c, false);
+    if (rc) return rc;
+    const size_t n = c->U * c->nr;
+    if (c->prec == PFC_FP64) {
+        HIPCHK(hipMemcpyAsync(fields, c->mreal, n * sizeof(double), hipMemcpyDeviceToHost,
+                              c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    } else {
+        std::vector<float> tmp(n);
+        HIPCHK(hipMemcpyAsync(tmp.data(), c->mreal, n * sizeof(float), hipMemcpyDeviceToHost,
+                              c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (size_t i = 0; i < n; i++) fields[i] = (double)tmp[i];
+    }
+    return PFC_OK;
+}
+
+int pfc_get_energy(pfc_ctx* c, double* f) {
+    if (!c || !f) return PFC_EINVAL;
+    HIPCHK(hipMemcpyAsync(f, c->f, c->nr * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return PFC_OK;
+}
+
+int pfc_nan_step(pfc_ctx* c, int* step) {
+    if (!c || !step) return PFC_EINVAL;
+    int v = INT_MAX;
+    HIPCHK(hipMemcpyAsync(&v, c->nan_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    *step = (v == INT_MAX) ? -1 : v;
+    return PFC_OK;
+}
+
+}  // extern "C"

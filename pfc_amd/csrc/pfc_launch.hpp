@@ -1,0 +1,249 @@
+// pfc_launch.hpp -- context layout and kernel launchers, shared by the
+// engine (pfc_engine.hip) and the per-precision instantiation units
+// (pfc_launch_f64.hip, pfc_launch_f32.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../../include/pfc_relax.h"
+#include "pfc_kernels.hpp"
+
+namespace pfc {
+
+constexpr int kEnergyBlocks = 1024;
+constexpr int kEnergyBS = 256;
+
+struct RowCfg {
+    int tpb, tr;
+};
+// row transforms have length H = Nx/2; TR rows share a block of TPB*TR lanes
+constexpr RowCfg row_cfg(int nx) {
+    switch (nx) {
+        case 64: return {8, 32};
+        case 128: return {8, 32};
+        case 256: return {16, 16};
+        case 512: return {32, 8};
+        case 1024: return {64, 4};
+        case 2048: return {128, 2};
+        case 4096: return {256, 1};
+        case 8192: return {512, 1};
+        default: return {0, 0};
+    }
+}
+struct ColCfg {
+    int tpb, tc;
+};
+constexpr ColCfg col_cfg(int ny) {
+    switch (ny) {
+        case 64: return {8, 32};
+        case 128: return {16, 16};
+        case 256: return {32, 8};
+        case 512: return {64, 4};
+        case 1024: return {128, 2};
+        case 2048: return {256, 1};
+        case 4096: return {512, 1};
+        case 8192: return {1024, 1};
+        default: return {0, 0};
+    }
+}
+
+inline bool size_ok(int n) { return row_cfg(n).tpb != 0; }
+
+}  // namespace pfc
+
+// Device context behind the opaque pfc_ctx of the C ABI.
+struct pfc_ctx {
+    int Nx, Ny, U, prec, nO;
+    size_t nk, nr;
+    size_t esz;  // bytes per complex element
+    pfc_params p;
+    double dx, dy;
+    pfc::DevParams hp;
+    pfc::DevParams* dprm;
+    void *I, *J, *K, *Nh, *O, *mreal, *R, *twx, *twy;
+    double *f, *partials, *sums;
+    int* nan_flag;
+    hipStream_t stream;
+    int have_fields, have_box, have_f;
+    // pass timing (pfc_timing): 4 events per recorded step
+    int timing;
+    int nrec;
+    int nev;
+    hipEvent_t* ev;
+};
+
+namespace pfc {
+
+// Launchers for one element precision T; explicitly instantiated for double
+// and float in their own translation units.
+template <typename T>
+struct Launch {
+    using C = cpx<T>;
+
+    template <int NX, int U, bool REF>
+    static void row_step(pfc_ctx* c, int t) {
+        constexpr RowCfg g = row_cfg(NX);
+        hipLaunchKernelGGL((k_row_step<T, NX, g.tpb, g.tr, U, REF>), dim3(c->Ny / g.tr),
+                           dim3(g.tpb * g.tr), 0, c->stream, (const C*)c->I, (C*)c->J, (C*)c->K,
+                           c->nk, c->Ny, (const C*)c->twx, c->dprm, c->nan_flag, t);
+    }
+    template <int NX, int U>
+    static void row_step_u(pfc_ctx* c, int t, bool ref) {
+        if (ref) row_step<NX, U, true>(c, t);
+        else row_step<NX, U, false>(c, t);
+    }
+    template <int NX>
+    static void row_step_n(pfc_ctx* c, int t, bool ref) {
+        switch (c->U) {
+            case 1: row_step_u<NX, 1>(c, t, ref); break;
+            case 2: row_step_u<NX, 2>(c, t, ref); break;
+            case 3: row_step_u<NX, 3>(c, t, ref); break;
+            case 4: row_step_u<NX, 4>(c, t, ref); break;
+        }
+    }
+    static void row_step_any(pfc_ctx* c, int t, bool ref) {
+        switch (c->Nx) {
+            case 64: row_step_n<64>(c, t, ref); break;
+            case 128: row_step_n<128>(c, t, ref); break;
+            case 256: row_step_n<256>(c, t, ref); break;
+            case 512: row_step_n<512>(c, t, ref); break;
+            case 1024: row_step_n<1024>(c, t, ref); break;
+            case 2048: row_step_n<2048>(c, t, ref); break;
+            case 4096: row_step_n<4096>(c, t, ref); break;
+            case 8192: row_step_n<8192>(c, t, ref); break;
+        }
+    }
+
+    template <int NY, int U, bool REF>
+    static void col_step(pfc_ctx* c, hipEvent_t mid) {
+        constexpr ColCfg g = col_cfg(NY);
+        const int H = c->Nx / 2;
+        hipLaunchKernelGGL((k_col0_step<T, NY, g.tpb, U, REF>), dim3(1), dim3(g.tpb), 0,
+                           c->stream, (const C*)c->J, (const C*)c->K, (C*)c->Nh, (C*)c->I,
+                           (C*)c->O, c->nk, H, (const C*)c->twy, c->dprm);
+        if (mid) (void)hipEventRecord(mid, c->stream);
+        hipLaunchKernelGGL((k_col_step<T, NY, g.tpb, g.tc, U, REF>), dim3(H / g.tc),
+                           dim3(g.tpb * g.tc), 0, c->stream, (const C*)c->J, (const C*)c->K,
+                           (C*)c->Nh, (C*)c->I, c->nk, (const C*)c->twy, c->dprm);
+    }
+    template <int NY, int U>
+    static void col_step_u(pfc_ctx* c, bool ref, hipEvent_t mid) {
+        if (ref) col_step<NY, U, true>(c, mid);
+        else col_step<NY, U, false>(c, mid);
+    }
+    template <int NY>
+    static void col_step_n(pfc_ctx* c, bool ref, hipEvent_t mid) {
+        switch (c->U) {
+            case 1: col_step_u<NY, 1>(c, ref, mid); break;
+            case 2: col_step_u<NY, 2>(c, ref, mid); break;
+            case 3: col_step_u<NY, 3>(c, ref, mid); break;
+            case 4: col_step_u<NY, 4>(c, ref, mid); break;
+        }
+    }
+    static void col_step_any(pfc_ctx* c, bool ref, hipEvent_t mid) {
+        switch (c->Ny) {
+            case 64: col_step_n<64>(c, ref, mid); break;
+            case 128: col_step_n<128>(c, ref, mid); break;
+            case 256: col_step_n<256>(c, ref, mid); break;
+            case 512: col_step_n<512>(c, ref, mid); break;
+            case 1024: col_step_n<1024>(c, ref, mid); break;
+            case 2048: col_step_n<2048>(c, ref, mid); break;
+            case 4096: col_step_n<4096>(c, ref, mid); break;
+            case 8192: col_step_n<8192>(c, ref, mid); break;
+        }
+    }
+
+    template <int NX>
+    static void row_c2r(pfc_ctx* c, const void* src, void* dst, void* K, int nbuf) {
+        constexpr RowCfg g = row_cfg(NX);
+        hipLaunchKernelGGL((k_row_c2r<T, NX, g.tpb, g.tr>), dim3(c->Ny / g.tr, nbuf),
+                           dim3(g.tpb * g.tr), 0, c->stream, (const C*)src, (T*)dst, (C*)K, c->nk,
+                           c->nr, c->Ny, (const C*)c->twx);
+    }
+    static void row_c2r_any(pfc_ctx* c, const void* src, void* dst, void* K, int nbuf) {
+        switch (c->Nx) {
+            case 64: row_c2r<64>(c, src, dst, K, nbuf); break;
+            case 128: row_c2r<128>(c, src, dst, K, nbuf); break;
+            case 256: row_c2r<256>(c, src, dst, K, nbuf); break;
+            case 512: row_c2r<512>(c, src, dst, K, nbuf); break;
+            case 1024: row_c2r<1024>(c, src, dst, K, nbuf); break;
+            case 2048: row_c2r<2048>(c, src, dst, K, nbuf); break;
+            case 4096: row_c2r<4096>(c, src, dst, K, nbuf); break;
+            case 8192: row_c2r<8192>(c, src, dst, K, nbuf); break;
+        }
+    }
+    template <int NX>
+    static void row_r2c(pfc_ctx* c, const void* src, void* dst, int nbuf) {
+        constexpr RowCfg g = row_cfg(NX);
+        hipLaunchKernelGGL((k_row_r2c<T, NX, g.tpb, g.tr>), dim3(c->Ny / g.tr, nbuf),
+                           dim3(g.tpb * g.tr), 0, c->stream, (const T*)src, (C*)dst, c->nk, c->nr,
+                           c->Ny, (const C*)c->twx, (T)(1.0 / NX));
+    }
+    static void row_r2c_any(pfc_ctx* c, const void* src, void* dst, int nbuf) {
+        switch (c->Nx) {
+            case 64: row_r2c<64>(c, src, dst, nbuf); break;
+            case 128: row_r2c<128>(c, src, dst, nbuf); break;
+            case 256: row_r2c<256>(c, src, dst, nbuf); break;
+            case 512: row_r2c<512>(c, src, dst, nbuf); break;
+            case 1024: row_r2c<1024>(c, src, dst, nbuf); break;
+            case 2048: row_r2c<2048>(c, src, dst, nbuf); break;
+            case 4096: row_r2c<4096>(c, src, dst, nbuf); break;
+            case 8192: row_r2c<8192>(c, src, dst, nbuf); break;
+        }
+    }
+    template <int NY>
+    static void col_fwd(pfc_ctx* c, const void* src, void* dst, int nbuf, double scale) {
+        constexpr ColCfg g = col_cfg(NY);
+        const int H = c->Nx / 2;
+        hipLaunchKernelGGL((k_col_fwd<T, NY, g.tpb, g.tc>), dim3(H / g.tc, nbuf),
+                           dim3(g.tpb * g.tc), 0, c->stream, (const C*)src, (C*)dst, c->nk,
+                           (const C*)c->twy, scale);
+    }
+    static void col_fwd_any(pfc_ctx* c, const void* src, void* dst, int nbuf, double scale) {
+        switch (c->Ny) {
+            case 64: col_fwd<64>(c, src, dst, nbuf, scale); break;
+            case 128: col_fwd<128>(c, src, dst, nbuf, scale); break;
+            case 256: col_fwd<256>(c, src, dst, nbuf, scale); break;
+            case 512: col_fwd<512>(c, src, dst, nbuf, scale); break;
+            case 1024: col_fwd<1024>(c, src, dst, nbuf, scale); break;
+            case 2048: col_fwd<2048>(c, src, dst, nbuf, scale); break;
+            case 4096: col_fwd<4096>(c, src, dst, nbuf, scale); break;
+            case 8192: col_fwd<8192>(c, src, dst, nbuf, scale); break;
+        }
+    }
+    template <int NY>
+    static void col_eval(pfc_ctx* c) {
+        constexpr ColCfg g = col_cfg(NY);
+        const int H = c->Nx / 2;
+        hipLaunchKernelGGL((k_col_eval<T, NY, g.tpb, g.tc>), dim3(H / g.tc, c->U),
+                           dim3(g.tpb * g.tc), 0, c->stream, (const C*)c->K, (C*)c->Nh, (C*)c->O,
+                           c->nk, H, (const C*)c->twy, c->dprm);
+    }
+    static void col_eval_any(pfc_ctx* c) {
+        switch (c->Ny) {
+            case 64: col_eval<64>(c); break;
+            case 128: col_eval<128>(c); break;
+            case 256: col_eval<256>(c); break;
+            case 512: col_eval<512>(c); break;
+            case 1024: col_eval<1024>(c); break;
+            case 2048: col_eval<2048>(c); break;
+            case 4096: col_eval<4096>(c); break;
+            case 8192: col_eval<8192>(c); break;
+        }
+    }
+    template <int U>
+    static void energy(pfc_ctx* c) {
+        hipLaunchKernelGGL((k_energy<T, U, kEnergyBS>), dim3(kEnergyBlocks), dim3(kEnergyBS), 0,
+                           c->stream, (const T*)c->mreal, (const T*)c->R, c->f, c->nr, c->partials,
+                           c->dprm);
+    }
+    static void energy_any(pfc_ctx* c) {
+        switch (c->U) {
+            case 1: energy<1>(c); break;
+            case 2: energy<2>(c); break;
+            case 3: energy<3>(c); break;
+            case 4: energy<4>(c); break;
+        }
+    }
+};
+
+}  // namespace pfc

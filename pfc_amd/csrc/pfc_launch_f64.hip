@@ -1,0 +1,4 @@
+// pfc_launch_f64.hip -- instantiates every double kernel and launcher.
+#include "pfc_launch.hpp"
+
+template struct pfc::Launch<double>;

@@ -1,0 +1,155 @@
+"""ctypes binding of libpfcgpu (include/pfc_relax.h).
+
+This is the host-side mirror used by the tests and bench.py; the pfc-relax
+binary binds the same C ABI directly.  Loading fails loudly when the built
+library is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .params import FP64, MIXED, PfcParams
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "lib", "libpfcgpu.so")
+BIN_PATH = os.path.join(PKG, "bin", "pfc-relax")
+
+# exported symbols of include/pfc_relax.h
+SYMBOLS = (
+    "pfc_create", "pfc_set_fields", "pfc_set_box", "pfc_step", "pfc_evaluate",
+    "pfc_get_fields", "pfc_get_energy", "pfc_nan_step", "pfc_timing", "pfc_timing_read",
+    "pfc_destroy", "pfc_strerror",
+)
+
+_lib = None
+
+
+class PfcError(RuntimeError):
+    def __init__(self, code, what):
+        self.code = code
+        super().__init__(f"{what}: {strerror(code)} ({code})")
+
+
+def load():
+    """Loads libpfcgpu.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: run `python -m pfc_amd.build` first")
+    L = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    dp = ctypes.POINTER(ctypes.c_double)
+    L.pfc_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(PfcParams), ctypes.c_int,
+                             ctypes.c_int, ctypes.c_int]
+    L.pfc_set_fields.argtypes = [vp, dp]
+    L.pfc_set_box.argtypes = [vp, ctypes.c_double, ctypes.c_double]
+    L.pfc_step.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.pfc_evaluate.argtypes = [vp, dp, dp]
+    L.pfc_get_fields.argtypes = [vp, dp]
+    L.pfc_get_energy.argtypes = [vp, dp]
+    L.pfc_nan_step.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
+    L.pfc_timing.argtypes = [vp, ctypes.c_int]
+    L.pfc_timing_read.argtypes = [vp, dp, ctypes.POINTER(ctypes.c_int)]
+    L.pfc_destroy.argtypes = [vp]
+    L.pfc_strerror.argtypes = [ctypes.c_int]
+    L.pfc_strerror.restype = ctypes.c_char_p
+    for s in SYMBOLS:
+        if s != "pfc_strerror":
+            getattr(L, s).restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def strerror(code):
+    return load().pfc_strerror(code).decode()
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Engine:
+    """One device context: U fields on an Nx x Ny grid, resident on the GPU."""
+
+    def __init__(self, params: PfcParams, Nx: int, Ny: int, dx: float, dy: float,
+                 precision: int = FP64):
+        L = load()
+        self.U = params.U
+        self.Nx, self.Ny = Nx, Ny
+        self.params = params
+        self.precision = precision
+        self._h = ctypes.c_void_p()
+        self._check(L.pfc_create(ctypes.byref(self._h), ctypes.byref(params), Nx, Ny, precision),
+                    "pfc_create")
+        self.set_box(dx, dy)
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise PfcError(rc, what)
+
+    def close(self):
+        if self._h:
+            load().pfc_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_box(self, dx, dy):
+        self.dx, self.dy = dx, dy
+        self._check(load().pfc_set_box(self._h, dx, dy), "pfc_set_box")
+
+    def set_fields(self, fields):
+        f = np.ascontiguousarray(fields, dtype=np.float64).reshape(self.U, self.Ny, self.Nx)
+        self._check(load().pfc_set_fields(self._h, _dp(f)), "pfc_set_fields")
+
+    def get_fields(self):
+        out = np.empty((self.U, self.Ny, self.Nx))
+        self._check(load().pfc_get_fields(self._h, _dp(out)), "pfc_get_fields")
+        return out
+
+    def get_energy(self):
+        out = np.empty((self.Ny, self.Nx))
+        self._check(load().pfc_get_energy(self._h, _dp(out)), "pfc_get_energy")
+        return out
+
+    def step(self, t0, n=1):
+        self._check(load().pfc_step(self._h, t0, n), "pfc_step")
+
+    def evaluate(self):
+        fave = ctypes.c_double()
+        nave = np.zeros(self.U)
+        self._check(load().pfc_evaluate(self._h, ctypes.byref(fave), _dp(nave)), "pfc_evaluate")
+        return fave.value, nave
+
+    def timing(self, enable=True):
+        self._check(load().pfc_timing(self._h, 1 if enable else 0), "pfc_timing")
+
+    def timing_read(self):
+        """Summed (row, column-0, column) pass milliseconds and step count."""
+        ms = np.zeros(3)
+        n = ctypes.c_int()
+        self._check(load().pfc_timing_read(self._h, _dp(ms), ctypes.byref(n)), "pfc_timing_read")
+        return ms, n.value
+
+    def nan_step(self):
+        s = ctypes.c_int()
+        self._check(load().pfc_nan_step(self._h, ctypes.byref(s)), "pfc_nan_step")
+        return s.value
+
+
+__all__ = ["Engine", "PfcError", "load", "strerror", "FP64", "MIXED", "LIB_PATH", "BIN_PATH",
+           "SYMBOLS"]

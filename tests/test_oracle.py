@@ -1,0 +1,301 @@
+"""Pins the CPU oracle with checks that do not restate its own formulas:
+the DFT definition (brute force in extended precision), numpy's independent
+FFT, the exact solutions of the linear and uniform problems under the
+first-order exponential integrator, closed-form free energies of cosine
+modes, the parabola-vertex arithmetic of the box optimiser, and invariants
+(mass conservation, refresh idempotence).
+
+Reference: src/pfc-relax.c (update_AB :900, step :1133, evaluate :987,
+optimize :1250, relax :1292) and the model definition at :9-17.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_lib as OL
+from oracle_lib import Oracle
+from pfc_amd import graphene_params, hetero_params, make_params
+
+
+# ------------------------------------------------------------------ FFT
+
+def brute_dft(x, sign):
+    n = x.size
+    xl = x.astype(np.clongdouble)
+    j = np.arange(n)
+    pi_l = 4 * np.arctan(np.longdouble(1))          # pi in extended precision
+    ang = sign * 2 * pi_l * (np.outer(j, j) % n).astype(np.longdouble) / n
+    W = np.cos(ang) + 1j * np.sin(ang)
+    return (W * xl[None, :]).sum(axis=1)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 6, 8, 12, 16, 24, 32, 48, 64, 96, 128, 192])
+@pytest.mark.parametrize("sign", [-1, 1])
+def test_fft1d_vs_brute_force(n, sign):
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    ref = brute_dft(x, sign)
+    got = OL.fft1d(x, sign)
+    err = np.max(np.abs(got - ref.astype(np.complex128))) / max(1.0, np.max(np.abs(ref)))
+    assert err < 4e-16 * max(1, math.log2(n)) * 2
+
+
+@pytest.mark.parametrize("n", [256, 384, 512, 768, 1024, 1536, 2048, 4096])
+def test_fft1d_vs_numpy(n):
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    assert np.max(np.abs(OL.fft1d(x, -1) - np.fft.fft(x))) < 1e-12 * math.sqrt(n)
+    assert np.max(np.abs(OL.fft1d(x, 1) - np.fft.ifft(x) * n)) < 1e-12 * math.sqrt(n)
+
+
+@pytest.mark.parametrize("Ny,Nx", [(8, 8), (12, 16), (16, 6), (64, 32), (48, 96)])
+def test_r2c_c2r_2d(Ny, Nx):
+    rng = np.random.default_rng(Nx * Ny)
+    a = rng.standard_normal((Ny, Nx))
+    X = OL.r2c2d(a)
+    assert X.shape == (Nx // 2 + 1, Ny)
+    assert np.max(np.abs(X - np.fft.rfft2(a).T)) < 1e-12
+    assert np.max(np.abs(OL.c2r2d(X, Nx) / (Nx * Ny) - a)) < 1e-14
+
+
+def test_c2r_ignores_antihermitian_part():
+    """The c2r of a half spectrum uses only the Hermitian part of the
+    kx = 0 and kx = Nx/2 columns (as FFTW's c2r and numpy's irfft2 do)."""
+    Ny, Nx = 16, 12
+    rng = np.random.default_rng(0)
+    X = rng.standard_normal((Nx // 2 + 1, Ny)) + 1j * rng.standard_normal((Nx // 2 + 1, Ny))
+    got = OL.c2r2d(X, Nx)
+    ref = np.fft.irfft2(X.T, s=(Ny, Nx)) * Nx * Ny
+    assert np.max(np.abs(got - ref)) < 1e-12
+    Y = X.copy()
+    for col in (0, Nx // 2):
+        Y[col] = 0.5 * (X[col] + np.conj(X[col][(-np.arange(Ny)) % Ny]))
+    assert np.max(np.abs(OL.c2r2d(Y, Nx) - got)) < 1e-12
+
+
+# ---------------------------------------------------- linear dynamics
+
+def k_of(ix, iy, Nx, Ny, dx, dy):
+    kx = 2 * math.pi * ix / (Nx * dx)
+    ky = 2 * math.pi * iy / (Ny * dy)
+    return kx * kx + ky * ky
+
+
+@pytest.mark.parametrize("conserved", [False, True])
+def test_linear_single_mode_growth(conserved):
+    """gamma = delta = lambda = 0: each Fourier mode is multiplied per step by
+    exp(L(k) dt) with L(k) = box(k) * (alpha + beta * prod_w(omega_w (nu_w^2 - k^2)^2 + phi_w)),
+    box = -1 (nonconserved) or -k^2 (conserved): the linear PFC dispersion."""
+    Nx, Ny, dx, dy, dt = 32, 24, 0.9, 1.1, 0.7
+    alpha, beta = -0.2, 1.3
+    nu = [1.0, 1.0 / 0.8]       # input-file 1/nu values
+    omega = [1.0, 0.5]
+    phi = [0.0, 0.2]
+    p = make_params(U=1, lam=[0.0], alpha=[[alpha]], beta=[[beta]], gamma=[[0.0]], delta=[0.0],
+                    epsilon=[[0.0]], sigma=0.3, nu=[nu], omega=[omega], phi=[phi], mu=[[0.0]],
+                    c=1.0 if conserved else 0.0, dt=dt)
+    modes = [(0, 0, 0.3), (3, 0, 0.2), (0, 5, -0.1), (2, -3, 0.15), (Nx // 2, 0, 0.05)]
+    y, x = np.meshgrid(np.arange(Ny), np.arange(Nx), indexing="ij")
+    f0 = sum(a * np.cos(2 * math.pi * (ix * x / Nx + iy * y / Ny)) for ix, iy, a in modes)
+    T = 13
+    o = Oracle(p, Nx, Ny, dx, dy)
+    o.set_fields(f0[None])
+    assert o.steps(1, T) == 0
+    exp_f = np.zeros_like(f0)
+    for ix, iy, a in modes:
+        k2 = k_of(ix, iy, Nx, Ny, dx, dy)
+        P = 1.0
+        for nw, om, ph in zip(nu, omega, phi):
+            P *= om * (1.0 / nw ** 2 - k2) ** 2 + ph
+        box = -k2 if conserved else -1.0
+        g = math.exp(box * (alpha + beta * P) * dt) ** T
+        exp_f += a * g * np.cos(2 * math.pi * (ix * x / Nx + iy * y / Ny))
+    assert np.max(np.abs(o.get_fields()[0] - exp_f)) < 1e-13
+
+
+def test_uniform_field_exponential_integrator():
+    """A uniform field obeys dn/dt = -(a0 n + N(n)), a0 = alpha + beta P(0),
+    N(n) = lambda + gamma n^2 + delta n^3; the scheme's first-order
+    exponential integrator gives n' = e^{-a0 dt} n + (e^{-a0 dt} - 1)/a0 N(n)."""
+    lam, alpha, beta, gamma, delta, dt = 0.03, -0.25, 1.0, 1.0, 1.0, 0.4
+    p = graphene_params(alpha=alpha, beta=beta, gamma=gamma, delta=delta, lam=lam, dt=dt)
+    Nx = Ny = 16
+    n = 0.21
+    o = Oracle(p, Nx, Ny, 0.8, 0.8)
+    o.set_fields(np.full((1, Ny, Nx), n))
+    a0 = alpha + beta * 1.0   # P(0) = (nu^2)^2 = 1
+    A0 = math.exp(-a0 * dt)
+    for t in range(20):
+        o.step(t + 1)
+        n = A0 * n + (A0 - 1.0) / a0 * (lam + gamma * n * n + delta * n ** 3)
+    assert np.max(np.abs(o.get_fields() - n)) < 1e-14
+
+
+def test_two_field_linear_coupling():
+    """Two linear fields coupled by alpha_uv (real space), beta_uv (mu^2 + lap)^2
+    and epsilon_uv G*G (G(k) = exp(-k^2/(2 sigma^2))): one mode evolves by the
+    2x2 exponential-integrator recurrence
+    [n0, n1]' = [[A0, (A0-1)/a0 c], [(A1-1)/a1 c, A1]] [n0, n1],
+    c = alpha_uv + beta_uv (mu^2 - k^2)^2 + epsilon_uv G(k)^2."""
+    Nx, Ny, dx, dy, dt = 32, 32, 0.8, 0.8, 0.5
+    al = [-0.3, 0.2]
+    be = [1.0, 0.7]
+    auv, buv, euv, mu, sigma = 0.05, 0.08, -0.3, 1.0 / 1.1, 0.4
+    p = make_params(U=2, lam=[0, 0], alpha=[[al[0], auv], [0, al[1]]],
+                    beta=[[be[0], buv], [0, be[1]]], gamma=[[0, 0], [0, 0]], delta=[0, 0],
+                    epsilon=[[0, euv], [0, 0]], sigma=sigma, nu=[[1.0], [1.0]],
+                    omega=[[1.0], [1.0]], phi=[[0.0], [0.0]], mu=[[0, mu], [0, 0]], c=0.0, dt=dt)
+    ix, iy = 4, 3
+    y, x = np.meshgrid(np.arange(Ny), np.arange(Nx), indexing="ij")
+    mode = np.cos(2 * math.pi * (ix * x / Nx + iy * y / Ny))
+    a = np.array([0.1, -0.05])
+    o = Oracle(p, Nx, Ny, dx, dy)
+    o.set_fields(np.stack([a[0] * mode, a[1] * mode]))
+    T = 9
+    o.steps(1, T)
+    k2 = k_of(ix, iy, Nx, Ny, dx, dy)
+    G2 = math.exp(-k2 / (2 * sigma ** 2)) ** 2
+    c = auv + buv * (1.0 / mu ** 2 - k2) ** 2 + euv * G2
+    A = []
+    Bn = []
+    for u in range(2):
+        au = al[u] + be[u] * (1.0 - k2) ** 2
+        Au = math.exp(-au * dt)
+        A.append(Au)
+        Bn.append((Au - 1.0) / au)
+    Mx = np.array([[A[0], Bn[0] * c], [Bn[1] * c, A[1]]])
+    amp = np.linalg.matrix_power(Mx, T) @ a
+    got = o.get_fields()
+    assert np.max(np.abs(got[0] - amp[0] * mode)) < 1e-14
+    assert np.max(np.abs(got[1] - amp[1] * mode)) < 1e-14
+
+
+def test_conserved_mass_and_refresh_idempotence():
+    p = hetero_params(c=1.0)
+    from pfc_amd.inputs import hetero_patchwork
+    f = hetero_patchwork(32, 32, 0.7, 0.7, domain=16, seed=1)
+    o = Oracle(p, 32, 32, 0.7, 0.7)
+    o.set_fields(f)
+    o.steps(0, 30)
+    assert np.allclose(o.get_fields().mean(axis=(1, 2)), f.mean(axis=(1, 2)), atol=1e-14, rtol=0)
+    # a refresh (t % 100 == 0) re-synchronises N to FFT(n)/NxNy: roundoff only
+    o1 = Oracle(p, 32, 32, 0.7, 0.7)
+    o2 = Oracle(p, 32, 32, 0.7, 0.7)
+    g = o.get_fields()
+    o1.set_fields(g)
+    o2.set_fields(g)
+    o1.step(100)
+    o2.step(101)
+    d = np.max(np.abs(o1.get_fields() - o2.get_fields()))
+    assert d < 1e-14
+
+
+# ------------------------------------------------------------- energy
+
+def P_of(k2, nu2s, omegas, phis):
+    P = 1.0
+    for n2, om, ph in zip(nu2s, omegas, phis):
+        P *= om * (n2 - k2) ** 2 + ph
+    return P
+
+
+def test_energy_uniform_and_single_mode():
+    lam, alpha, beta, gamma, delta = 0.02, -0.25, 1.1, 0.9, 1.2
+    p = graphene_params(alpha=alpha, beta=beta, gamma=gamma, delta=delta, lam=lam)
+    Nx, Ny, dx, dy = 64, 32, 0.85, 0.9
+    o = Oracle(p, Nx, Ny, dx, dy)
+    nb = 0.13
+    o.set_fields(np.full((1, Ny, Nx), nb))
+    fave, nave = o.evaluate()
+    f_exp = lam * nb + 0.5 * (alpha + beta * 1.0) * nb ** 2 + gamma * nb ** 3 / 3 + delta * nb ** 4 / 4
+    assert abs(fave - f_exp) < 1e-15
+    assert abs(nave[0] - nb) < 1e-16
+    # n = nb + A cos(q x): <cos^2> = 1/2, <cos^3> = 0, <cos^4> = 3/8
+    ix = 7
+    A = 0.3
+    x = np.arange(Nx)
+    field = nb + A * np.cos(2 * math.pi * ix * x / Nx)[None, :] * np.ones((Ny, 1))
+    o.set_fields(field[None])
+    fave, nave = o.evaluate()
+    q2 = k_of(ix, 0, Nx, Ny, dx, dy)
+    Pq = (1.0 - q2) ** 2
+    f_exp = (lam * nb + 0.5 * alpha * (nb ** 2 + A * A / 2) + 0.5 * beta * (nb ** 2 + Pq * A * A / 2)
+             + gamma / 3 * (nb ** 3 + 1.5 * nb * A * A)
+             + delta / 4 * (nb ** 4 + 3 * nb * nb * A * A + 3 * A ** 4 / 8))
+    assert abs(fave - f_exp) < 1e-14
+    assert abs(nave[0] - nb) < 1e-15
+
+
+@pytest.mark.parametrize("compat", [0, 1])
+def test_energy_two_field_pair_terms(compat):
+    """Pair terms for two cosine modes of equal wavevector:
+    alpha_uv <n_u n_v>, beta_uv <n_u (mu^2+lap)^2 n_v>, gamma_uv/2 <n_u^2 n_v + n_u n_v^2>,
+    epsilon_uv <(G*n_u)(G*n_v)>."""
+    Nx, Ny, dx, dy = 32, 32, 0.8, 0.8
+    auv, buv, guv, euv, mu, sigma = 0.07, 0.04, 0.3, -0.6, 1.0 / 1.05, 0.35
+    p = make_params(U=2, lam=[0.0, 0.0], alpha=[[0.0, auv], [0, 0.0]], beta=[[0.0, buv], [0, 0.0]],
+                    gamma=[[0.0, guv], [0, 0.0]], delta=[0.0, 0.0], epsilon=[[0, euv], [0, 0]],
+                    sigma=sigma, nu=[[1.0], [1.0]], omega=[[1.0], [1.0]], phi=[[0.0], [0.0]],
+                    mu=[[0, mu], [0, 0]], c=0.0, dt=0.1, energy_compat=compat)
+    ix, iy = 3, 2
+    y, x = np.meshgrid(np.arange(Ny), np.arange(Nx), indexing="ij")
+    cs = np.cos(2 * math.pi * (ix * x / Nx + iy * y / Ny))
+    a, A, b, B = 0.1, 0.2, -0.05, 0.15
+    o = Oracle(p, Nx, Ny, dx, dy)
+    o.set_fields(np.stack([a + A * cs, b + B * cs]))
+    fave, nave = o.evaluate()
+    k2 = k_of(ix, iy, Nx, Ny, dx, dy)
+    # the reference's evaluate reads nu2[u][v] where the model has mu2; nu2[0][1]
+    # lies past the end of nu2[0] (W = 1) and the compat mode uses 0 there
+    m2 = 1.0 / mu ** 2 if compat == 0 else 0.0
+    G2 = math.exp(-k2 / (2 * sigma ** 2)) ** 2
+    f_exp = (auv * (a * b + A * B / 2)
+             + buv * (m2 ** 2 * a * b + (m2 - k2) ** 2 * A * B / 2)
+             + 0.5 * guv * (a * a * b + A * A * b / 2 + a * A * B + a * b * b + a * B * B / 2 + b * A * B)
+             + euv * (a * b + G2 * A * B / 2))
+    assert abs(fave - f_exp) < 1e-15
+    assert np.allclose(nave, [a, b], atol=1e-16)
+
+
+# ----------------------------------------------------------- optimise
+
+def test_optimize_parabola_vertex():
+    """For a quadratic energy the 3-point parabola vertex is exact."""
+    xs, ys = 0.7012, 0.6989
+    f = lambda dx, dy: 1.5 + 3.0 * (dx - xs) ** 2 + 7.0 * (dy - ys) ** 2
+    dx, dy, d = OL.optimize_with(64, 64, 0.7, 0.7, 1e-4, 1, 1, f)
+    # second differences of O(d^2) on O(1) energies: ~1e-12 roundoff
+    assert abs(dx - xs) < 1e-10 and abs(dy - ys) < 1e-10
+    # |change| = sqrt(0.0012^2 + 0.0011^2) = 1.6e-3 >= d -> d doubles
+    assert d == 2e-4
+    # only x optimised: dy stays
+    dx, dy, d = OL.optimize_with(64, 64, 0.7, 0.7, 1e-4, 1, 0, f)
+    assert abs(dx - xs) < 1e-10 and dy == 0.7
+
+
+def test_optimize_limiter_and_step_floor():
+    """Size changes are truncated to a quarter of l0 = 4 pi/sqrt(3)."""
+    Nx, Ny = 128, 64
+    xs = 0.75
+    f = lambda dx, dy: (dx - xs) ** 2 + (dy - 0.7) ** 2
+    dx, dy, d = OL.optimize_with(Nx, Ny, 0.7, 0.7, 1e-3, 1, 1, f)
+    l0 = 4 * math.pi / math.sqrt(3)
+    assert abs(Nx * (dx - 0.7) - 0.25 * l0) < 1e-12
+    assert abs(dy - 0.7) < 1e-15
+    # no change -> d halves; below 1e-6 it is doubled back
+    g = lambda dx, dy: (dx - 0.7) ** 2 + (dy - 0.7) ** 2
+    _, _, d = OL.optimize_with(Nx, Ny, 0.7, 0.7, 1e-4, 1, 1, g)
+    assert d == 5e-5
+    _, _, d = OL.optimize_with(Nx, Ny, 0.7, 0.7, 1.5e-6, 1, 1, g)
+    assert d == 1.5e-6
+
+
+def test_relax_cadence():
+    p = graphene_params()
+    o = Oracle(p, 32, 32, 0.9, 0.9)
+    rng = np.random.default_rng(0)
+    o.set_fields(0.2 + 0.05 * rng.standard_normal((1, 32, 32)))
+    log, snaps = o.relax(tend=250, Tprint=100, Topt=0, snap_steps=(0, 100))
+    assert list(log[:, 0]) == [0, 100, 200]
+    assert snaps.shape == (2, 1, 32, 32)
