@@ -111,11 +111,11 @@ int pfc_get_energy(pfc_ctx* ctx, double* f);
 int pfc_nan_step(pfc_ctx* ctx, int* step);
 
 /* Measurement hooks; no counterpart in the reference (it only prints whole
- * seconds, :880).  While enabled, pfc_step brackets each of its passes with
- * HIP events on the context's stream.  pfc_timing(ctx, 1) also clears what
- * was collected.  pfc_timing_read waits for the stream and returns the summed
- * milliseconds of the row pass (ms[0]), the packed-column-0 pass (ms[1]) and
- * the column pass (ms[2]) over *nsteps recorded steps. */
+ * seconds, :880).  While enabled, pfc_step brackets each of its two passes
+ * with HIP events on the context's stream.  pfc_timing(ctx, 1) also clears
+ * what was collected.  pfc_timing_read waits for the stream and returns the
+ * summed milliseconds of the row pass (ms[0]) and the column pass (ms[1])
+ * over *nsteps recorded steps. */
 int pfc_timing(pfc_ctx* ctx, int enable);
 int pfc_timing_read(pfc_ctx* ctx, double* ms, int* nsteps);
 

@@ -44,20 +44,23 @@ def _run(cmd):
     return out
 
 
-def build_lib(verbose=False):
-    os.makedirs(OBJDIR, exist_ok=True)
-    os.makedirs(LIBDIR, exist_ok=True)
+def build_lib(verbose=False, libdir=None, objdir=None, defines=()):
+    libdir = libdir or LIBDIR
+    objdir = objdir or OBJDIR
+    os.makedirs(objdir, exist_ok=True)
+    os.makedirs(libdir, exist_ok=True)
     objs = []
     cmds = []
     for u in HIP_UNITS:
-        obj = os.path.join(OBJDIR, u.replace(".hip", ".o"))
+        obj = os.path.join(objdir, u.replace(".hip", ".o"))
         objs.append(obj)
-        cmds.append([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, u), "-o", obj])
+        cmds.append([HIPCC, *HIP_FLAGS, *[f"-D{d}" for d in defines], "-c", os.path.join(CSRC, u),
+                     "-o", obj])
     with cf.ThreadPoolExecutor(max_workers=len(cmds)) as ex:
         for out in ex.map(_run, cmds):
             if verbose and out.strip():
                 print(out)
-    lib = os.path.join(LIBDIR, "libpfcgpu.so")
+    lib = os.path.join(libdir, "libpfcgpu.so")
     tmp = lib + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, lib)

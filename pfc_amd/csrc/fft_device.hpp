@@ -14,6 +14,13 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#ifndef PFC_TW_RECURRENCE
+#define PFC_TW_RECURRENCE 1
+#endif
+#ifndef PFC_STORE_B128
+#define PFC_STORE_B128 0
+#endif
+
 namespace pfc {
 
 template <typename T>
@@ -58,6 +65,32 @@ __host__ __device__ constexpr double c16(int k) {
 }
 __host__ __device__ constexpr double s16(int k) { return c16(k - 4); }
 
+// cos/sin(2 pi k / 32), k = 0..31 (non-recursive so it folds after unrolling)
+__host__ __device__ constexpr double c32(int k) {
+    constexpr double t[9] = {1.0,
+                             0.98078528040323044913,
+                             0.92387953251128675613,
+                             0.83146961230254523708,
+                             0.70710678118654752440,
+                             0.55557023301960222474,
+                             0.38268343236508977173,
+                             0.19509032201612826785,
+                             0.0};
+    k &= 31;
+    if (k <= 8) return t[k];
+    if (k <= 16) return -t[16 - k];
+    if (k <= 24) return -t[k - 16];
+    return t[32 - k];
+}
+__host__ __device__ constexpr double s32(int k) { return c32(k - 8); }
+
+// exp(DIR 2 pi i k / n) for n dividing 32, folded at compile time
+template <typename T>
+__device__ __forceinline__ cpx<T> root(int k, int n, int dir) {
+    const int m = k * (32 / n);
+    return {(T)c32(m), (T)(dir * s32(m))};
+}
+
 // In-register DFT of size R (1, 2, 4, 8, 16): v[k] <- sum_r v[r] e^{DIR 2 pi i rk/R}
 template <int R, int DIR, typename T>
 __device__ __forceinline__ void dft(cpx<T>* v) {
@@ -95,6 +128,75 @@ __device__ __forceinline__ void dft(cpx<T>* v) {
     }
 }
 
+// ------------------------------------------------ buffer-resource access
+//
+// Global memory is addressed through buffer resources: a 128-bit descriptor
+// in SGPRs, a 32-bit per-lane byte offset (one VGPR shared by every access
+// of the same lane pattern) and a uniform byte offset in an SGPR.  This keeps
+// 64-bit per-access address pairs out of the register file.
+
+struct GBuf {
+    __amdgpu_buffer_rsrc_t r;
+};
+__device__ __forceinline__ GBuf gbuf(const void* p) {
+    // num_records = 0xffffffff (whole 4 GiB window), gfx9 word 3 = 0x00020000
+    return {__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)0xffffffff,
+                                             0x00020000)};
+}
+
+template <typename T>
+__device__ __forceinline__ cpx<T> gload(GBuf b, unsigned voff, unsigned soff);
+template <>
+__device__ __forceinline__ cpx<double> gload<double>(GBuf b, unsigned voff, unsigned soff) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(b.r, (int)voff, (int)soff, 0);
+    return __builtin_bit_cast(cpx<double>, v);
+}
+template <>
+__device__ __forceinline__ cpx<float> gload<float>(GBuf b, unsigned voff, unsigned soff) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b64(b.r, (int)voff, (int)soff, 0);
+    return __builtin_bit_cast(cpx<float>, v);
+}
+template <typename T>
+__device__ __forceinline__ void gstore(GBuf b, unsigned voff, unsigned soff, cpx<T> x);
+template <>
+__device__ __forceinline__ void gstore<double>(GBuf b, unsigned voff, unsigned soff,
+                                               cpx<double> x) {
+#if PFC_STORE_B128
+    using V = decltype(__builtin_amdgcn_raw_buffer_load_b128(b.r, 0, 0, 0));
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(V, x), b.r, (int)voff, (int)soff, 0);
+#else
+    // Two 64-bit stores of the same bytes instead of one 128-bit store: a
+    // 128-bit buffer store with an SGPR soffset whose data VGPRs are rewritten
+    // by the next VALU instructions has stored corrupted upper halves on
+    // gfx950 (the compiler inserts no wait states for that case).  The second
+    // half's address goes through an opaque register copy so the backend
+    // cannot merge the pair back into a 128-bit store.
+    using V = decltype(__builtin_amdgcn_raw_buffer_load_b64(b.r, 0, 0, 0));
+    unsigned v2 = voff + 8u;
+    asm volatile("" : "+v"(v2));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V, x.x), b.r, (int)voff, (int)soff, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V, x.y), b.r, (int)v2, (int)soff, 0);
+#endif
+}
+template <>
+__device__ __forceinline__ void gstore<float>(GBuf b, unsigned voff, unsigned soff, cpx<float> x) {
+    using V = decltype(__builtin_amdgcn_raw_buffer_load_b64(b.r, 0, 0, 0));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V, x), b.r, (int)voff, (int)soff, 0);
+}
+
+// one real value (the real or imaginary half of a complex element)
+template <typename T>
+__device__ __forceinline__ void gstore_real(GBuf b, unsigned voff, unsigned soff, T x);
+template <>
+__device__ __forceinline__ void gstore_real<double>(GBuf b, unsigned voff, unsigned soff, double x) {
+    using V = decltype(__builtin_amdgcn_raw_buffer_load_b64(b.r, 0, 0, 0));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V, x), b.r, (int)voff, (int)soff, 0);
+}
+template <>
+__device__ __forceinline__ void gstore_real<float>(GBuf b, unsigned voff, unsigned soff, float x) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, x), b.r, (int)voff, (int)soff, 0);
+}
+
 constexpr int ilog(int b, int n) { return n < b ? 0 : 1 + ilog(b, n / b); }
 constexpr int ipow(int b, int e) { return e == 0 ? 1 : b * ipow(b, e - 1); }
 
@@ -109,16 +211,16 @@ struct BlockFFT {
     static constexpr int S = ilog(P, L);          // number of radix-P stages
     static constexpr int R0 = L / ipow(P, S);     // leading stage radix (1 if none)
 
-    __device__ static __forceinline__ cpx<T> twid(const cpx<T>* __restrict__ tw, int m) {
-        cpx<T> w = tw[m * TWS];
+    __device__ static __forceinline__ cpx<T> twid(GBuf tw, int m) {
+        cpx<T> w = gload<T>(tw, (unsigned)(m * TWS) * sizeof(cpx<T>), 0u);
         if constexpr (DIR > 0) w.y = -w.y;
         return w;
     }
 
     // One radix-R Stockham stage on butterflies j = t + b*TPB, inputs in in[].
     template <int R, bool LAST>
-    __device__ static __forceinline__ void stage(cpx<T> (&in)[P], int Ns, cpx<T>* lds,
-                                                 const cpx<T>* __restrict__ tw, int t) {
+    __device__ static __forceinline__ void stage(cpx<T> (&in)[P], int Ns, cpx<T>* lds, GBuf tw,
+                                                 int t) {
         constexpr int B = P / R;
 #pragma unroll
         for (int b = 0; b < B; b++) {
@@ -129,8 +231,37 @@ struct BlockFFT {
             for (int r = 0; r < R; r++) x[r] = in[b + r * B];
             if (Ns > 1) {
                 const int step = L / (Ns * R);
+                if constexpr (sizeof(T) == 8 && PFC_TW_RECURRENCE) {
+                    // FP64: powers w^r of one table twiddle by recurrence
+                    // (error grows by ~r ulp, r < 16), far fewer loads and
+                    // live registers than one load per r
+                    const cpx<T> w1 = twid(tw, k * step);
+                    cpx<T> w = w1;
 #pragma unroll
-                for (int r = 1; r < R; r++) x[r] = cmul(x[r], twid(tw, k * r * step));
+                    for (int r = 1; r < R; r++) {
+                        x[r] = cmul(x[r], w);
+                        if (r + 1 < R) w = cmul(w, w1);
+                    }
+                } else if constexpr (R >= 4) {
+                    // FP32: two table twiddles w and w^(R/2); every other
+                    // power is at most two products away (~2 ulp)
+                    constexpr int Hh = R / 2;
+                    const cpx<T> w1 = twid(tw, k * step);
+                    const cpx<T> wh = twid(tw, k * Hh * step);
+                    cpx<T> lo[Hh];
+                    lo[0] = {(T)1, (T)0};
+                    lo[1] = w1;
+#pragma unroll
+                    for (int r = 2; r < Hh; r++) lo[r] = cmul(lo[r - 1], w1);
+#pragma unroll
+                    for (int r = 1; r < R; r++) {
+                        const cpx<T> w = (r < Hh) ? lo[r] : (r == Hh ? wh : cmul(wh, lo[r - Hh]));
+                        x[r] = cmul(x[r], w);
+                    }
+                } else {
+#pragma unroll
+                    for (int r = 1; r < R; r++) x[r] = cmul(x[r], twid(tw, k * r * step));
+                }
             }
             dft<R, DIR, T>(x);
             if constexpr (LAST) {
@@ -158,8 +289,7 @@ struct BlockFFT {
 
     // v: natural-order input (thread t holds t + q*TPB); same layout on exit.
     // Begins with a barrier so lds may still be in use by the caller.
-    __device__ static __forceinline__ void run(cpx<T> (&v)[P], cpx<T>* lds,
-                                               const cpx<T>* __restrict__ tw, int t) {
+    __device__ static __forceinline__ void run(cpx<T> (&v)[P], cpx<T>* lds, GBuf tw, int t) {
         int Ns = 1;
         __syncthreads();
         if constexpr (R0 > 1) {

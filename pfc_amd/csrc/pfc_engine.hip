@@ -2,7 +2,8 @@
 //
 // Device state of a context (E = complex element, FP64 or FP32 by precision;
 // nk = (Nx/2)*Ny, nr = Nx*Ny):
-//   I, J, K, Nh   U*nk E each  (layout in pfc_kernels.hpp)
+//   I, J, K       U*nk E each, nk = (Nx/2)*Ny (packed row transforms)
+//   Nh            U*nkn E, nkn = (Nx/2+1)*Ny (the spectral state)
 //   O             (2U + U(U-1)/2)*nk E   evaluate's k-space outputs
 //   mreal         U*nr real    densities in real space (materialised on demand)
 //   R             (2U + U(U-1)/2)*nr real  evaluate's real-space outputs
@@ -99,11 +100,11 @@ void free_all(pfc_ctx* c) {
     if (c->stream) (void)hipStreamDestroy(c->stream);
 }
 
-// Grows the event pool to hold n recorded steps (4 events each).
+// Grows the event pool to hold n recorded steps (3 events each).
 int ensure_events(pfc_ctx* c, int n) {
-    if (4 * n <= c->nev) return PFC_OK;
+    if (3 * n <= c->nev) return PFC_OK;
     int cap = c->nev ? c->nev : 64;
-    while (cap < 4 * n) cap *= 2;
+    while (cap < 3 * n) cap *= 2;
     hipEvent_t* ev = (hipEvent_t*)realloc(c->ev, sizeof(hipEvent_t) * cap);
     if (!ev) return PFC_ENOMEM;
     for (int i = c->nev; i < cap; i++)
@@ -172,12 +173,14 @@ int pfc_create(pfc_ctx** out, const pfc_params* p, int Nx, int Ny, int precision
     c->p = *p;
     c->nO = 2 * c->U + c->U * (c->U - 1) / 2;
     c->nk = (size_t)(Nx / 2) * Ny;
+    c->nkn = (size_t)(Nx / 2 + 1) * Ny;   // half spectrum, kx = 0..Nx/2
     c->nr = (size_t)Nx * Ny;
     c->esz = (precision == PFC_FP64) ? 16 : 8;
     const size_t rsz = c->esz / 2;
     c->dx = 1.0;
     c->dy = 1.0;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        free_all(c);
         free(c);
         return PFC_EDEVICE;
     }
@@ -186,7 +189,7 @@ int pfc_create(pfc_ctx** out, const pfc_params* p, int Nx, int Ny, int precision
     rc |= dalloc(&c->I, spec);
     rc |= dalloc(&c->J, spec);
     rc |= dalloc(&c->K, spec);
-    rc |= dalloc(&c->Nh, spec);
+    rc |= dalloc(&c->Nh, c->U * c->nkn * c->esz);
     rc |= dalloc(&c->O, c->nO * c->nk * c->esz);
     rc |= dalloc(&c->mreal, c->U * c->nr * rsz);
     rc |= dalloc(&c->R, c->nO * c->nr * rsz);
@@ -277,15 +280,15 @@ int pfc_step(pfc_ctx* c, int t0, int nsteps) {
         hipEvent_t* ev = nullptr;
         if (c->timing) {
             if (ensure_events(c, c->nrec + 1) != PFC_OK) return PFC_EDEVICE;
-            ev = c->ev + 4 * c->nrec++;
+            ev = c->ev + 3 * c->nrec++;
             HIPCHK(hipEventRecord(ev[0], c->stream));
         }
         if (c->prec == PFC_FP64) Launch<double>::row_step_any(c, t, ref);
         else Launch<float>::row_step_any(c, t, ref);
         if (ev) HIPCHK(hipEventRecord(ev[1], c->stream));
-        if (c->prec == PFC_FP64) Launch<double>::col_step_any(c, ref, ev ? ev[2] : nullptr);
-        else Launch<float>::col_step_any(c, ref, ev ? ev[2] : nullptr);
-        if (ev) HIPCHK(hipEventRecord(ev[3], c->stream));
+        if (c->prec == PFC_FP64) Launch<double>::col_step_any(c, ref);
+        else Launch<float>::col_step_any(c, ref);
+        if (ev) HIPCHK(hipEventRecord(ev[2], c->stream));
     }
     HIPCHK(hipGetLastError());
     return PFC_OK;
@@ -301,10 +304,10 @@ int pfc_timing(pfc_ctx* c, int enable) {
 int pfc_timing_read(pfc_ctx* c, double* ms, int* nsteps) {
     if (!c || !ms || !nsteps) return PFC_EINVAL;
     HIPCHK(hipStreamSynchronize(c->stream));
-    ms[0] = ms[1] = ms[2] = 0.0;
+    ms[0] = ms[1] = 0.0;
     for (int i = 0; i < c->nrec; i++) {
-        hipEvent_t* e = c->ev + 4 * i;
-        for (int k = 0; k < 3; k++) {
+        hipEvent_t* e = c->ev + 3 * i;
+        for (int k = 0; k < 2; k++) {
             float v = 0.f;
             HIPCHK(hipEventElapsedTime(&v, e[k], e[k + 1]));
             ms[k] += v;

@@ -1,20 +1,22 @@
 // pfc_kernels.hpp -- gfx950 kernels of the pfc-relax step and evaluation.
 //
-// Spectral-side layout (all device buffers of one field are H x Ny complex,
-// H = Nx/2, stored column-major: element (kx, y) at kx*Ny + y):
-//   I  -- row transforms of the density, R2C_x(n)(kx, y), the step state
-//   J  -- row transforms of the nonlinear term
+// Spectral-side layout.  Row-spectral buffers of one field (I, J, K, O) hold
+// H = Nx/2 columns of Ny complex values, column-major (element (kx, y) at
+// kx*Ny + y); the R2C of a real row has real kx = 0 and kx = H entries, so
+// column 0 carries them as (DC, Nyquist) = (re, im).  The spectral state Nh
+// holds the half spectrum's H+1 columns unpacked (kx = 0..H), the transposed
+// layout of the reference's FFTW_MPI_TRANSPOSED_OUT transforms
+// (src/pfc-relax.c:227-235):
+//   I  -- row transforms of the density, R2C_x(n)(kx, y)/Nx, the step state
+//   J  -- row transforms of the nonlinear term (unnormalised)
 //   Nh -- the spectral state N(kx, ky)
-// The half spectrum has H+1 columns; the two real-along-y columns kx = 0 and
-// kx = H are packed into column 0 as (DC) + i (Nyquist), so every buffer and
-// every column transform is a power of two.  In k space column 0 therefore
-// holds P(ky) = N0(ky) + i NH(ky) for the two Hermitian spectra N0, NH.
-//
 // One step (reference step(), src/pfc-relax.c:1133-1247) is two kernels:
 //   k_row_step  per row y:   I -> C2R_x -> n -> nonlinear -> R2C_x -> J
 //   k_col_step  per column:  J -> FFT_y -> M; update with Nh -> Nh'; IFFT_y -> I
 // so each field costs 16 B (row) + 32 B (column) of HBM traffic per point in
-// FP64, half that in FP32.
+// FP64, half that in FP32.  Block 0 of a column pass runs the kx = 0 and
+// kx = H columns one after the other from the packed column 0, keeping only
+// the real parts of their inverse transforms, which is what a 1D c2r uses.
 #pragma once
 #include "fft_device.hpp"
 
@@ -137,9 +139,10 @@ __device__ __forceinline__ void kupdate(const DevParams* __restrict__ p, T k2, c
 // X[t + q*TPB], X[0] packed as (DC, Nyquist)): produces Z with
 // IFFT_H(Z)[n] = x[2n] + i x[2n+1], x the unnormalised c2r of X.
 template <typename T, int H, int TPB, int LS>
-__device__ __forceinline__ void c2r_pre(cpx<T> (&v)[H / TPB], cpx<T>* lds,
-                                        const cpx<T>* __restrict__ twx, int t) {
+__device__ __forceinline__ void c2r_pre(cpx<T> (&v)[H / TPB], cpx<T>* lds, GBuf twx, int t) {
     constexpr int P = H / TPB;
+    // e^{-2 pi i k/Nx} for k = t + q*TPB is e^{-2 pi i t/Nx} e^{-2 pi i q/(2P)}
+    const cpx<T> wt = gload<T>(twx, (unsigned)t * sizeof(cpx<T>), 0u);
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < P; q++) lds[(t + q * TPB) * LS] = v[q];
@@ -148,13 +151,14 @@ __device__ __forceinline__ void c2r_pre(cpx<T> (&v)[H / TPB], cpx<T>* lds,
     for (int q = 0; q < P; q++) {
         const int k = t + q * TPB;
         if (k == 0) {
-            const T x0 = v[q].x, xh = v[q].y;
+            const T x0 = v[q].x, xh = v[q].y;   // packed DC and Nyquist
             v[q] = {x0 + xh, x0 - xh};
         } else {
             const cpx<T> a = v[q];
             const cpx<T> b = cconj(lds[(H - k) * LS]);
             const cpx<T> xe = cadd(a, b);
-            const cpx<T> xo = cmulc(csub(a, b), twx[k]);   // * e^{+2 pi i k/Nx}
+            const cpx<T> w = cmul(wt, root<T>(q, 2 * P, -1));   // e^{-2 pi i k/Nx}
+            const cpx<T> xo = cmulc(csub(a, b), w);   // * e^{+2 pi i k/Nx}
             v[q] = {xe.x - xo.y, xe.y + xo.x};              // xe + i xo
         }
     }
@@ -163,9 +167,9 @@ __device__ __forceinline__ void c2r_pre(cpx<T> (&v)[H / TPB], cpx<T>* lds,
 // R2C post-processing: v holds Z = FFT_H(z), z[n] = x[2n] + i x[2n+1], in
 // natural order; produces X[k] (k < H) with X[0] packed as (DC, Nyquist).
 template <typename T, int H, int TPB, int LS>
-__device__ __forceinline__ void r2c_post(cpx<T> (&v)[H / TPB], cpx<T>* lds,
-                                         const cpx<T>* __restrict__ twx, int t) {
+__device__ __forceinline__ void r2c_post(cpx<T> (&v)[H / TPB], cpx<T>* lds, GBuf twx, int t) {
     constexpr int P = H / TPB;
+    const cpx<T> wt = gload<T>(twx, (unsigned)t * sizeof(cpx<T>), 0u);
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < P; q++) lds[(t + q * TPB) * LS] = v[q];
@@ -175,14 +179,15 @@ __device__ __forceinline__ void r2c_post(cpx<T> (&v)[H / TPB], cpx<T>* lds,
         const int k = t + q * TPB;
         if (k == 0) {
             const T re = v[q].x, im = v[q].y;
-            v[q] = {re + im, re - im};
+            v[q] = {re + im, re - im};   // packed (DC, Nyquist)
         } else {
             const cpx<T> a = v[q];
             const cpx<T> b = cconj(lds[(H - k) * LS]);
             const cpx<T> s = cadd(a, b);   // 2 Xe
             const cpx<T> d = csub(a, b);   // 2i Xo
             const cpx<T> xo2 = {d.y, -d.x};   // 2 Xo
-            const cpx<T> r = cadd(s, cmul(xo2, twx[k]));
+            const cpx<T> w = cmul(wt, root<T>(q, 2 * P, -1));   // e^{-2 pi i k/Nx}
+            const cpx<T> r = cadd(s, cmul(xo2, w));
             v[q] = cscale(r, (T)0.5);
         }
     }
@@ -217,14 +222,16 @@ __device__ __forceinline__ void nonlinear(const DevParams* __restrict__ p, T (&m
 
 // Row pass of a step.  Block = TR rows, lane = (fft thread t, row r) with the
 // row index fastest, so the TR rows' elements of one kx are adjacent lanes
-// and adjacent addresses in the column-major buffers.
+// and adjacent addresses in the column-major buffers (TR*sizeof(E) bytes
+// contiguous per kx).
 template <typename T, int NX, int TPB, int TR, int U, bool REFRESH>
 __global__ __launch_bounds__(TPB* TR) PFC_WAVES(PFC_ROW_WAVES) void k_row_step(
     const cpx<T>* __restrict__ I, cpx<T>* __restrict__ J, cpx<T>* __restrict__ K,
-    size_t fstride, int Ny, const cpx<T>* __restrict__ twx, const DevParams* __restrict__ prm,
+    size_t fstride, int Ny, const cpx<T>* __restrict__ twx_, const DevParams* __restrict__ prm,
     int* __restrict__ nan_step, int tstep) {
     constexpr int H = NX / 2;
     constexpr int P = H / TPB;
+    constexpr unsigned E = sizeof(cpx<T>);
     using FFTF = BlockFFT<T, H, TPB, -1, 2, TR>;
     using FFTI = BlockFFT<T, H, TPB, +1, 2, TR>;
     __shared__ cpx<T> lds[H * TR];
@@ -233,13 +240,16 @@ __global__ __launch_bounds__(TPB* TR) PFC_WAVES(PFC_ROW_WAVES) void k_row_step(
     const int t = tid / TR;
     const int y = xcd_remap(blockIdx.x, gridDim.x) * TR + r;
     cpx<T>* L = lds + r;
+    const GBuf twx = gbuf(twx_);
+    const unsigned voff = ((unsigned)t * (unsigned)Ny + (unsigned)y) * E;
+    const unsigned qstep = (unsigned)TPB * (unsigned)Ny * E;
 
     cpx<T> m[U][P];
 #pragma unroll
     for (int u = 0; u < U; u++) {
-        const cpx<T>* src = I + u * fstride + y;
+        const GBuf src = gbuf(I + u * fstride);
 #pragma unroll
-        for (int q = 0; q < P; q++) m[u][q] = src[(size_t)(t + q * TPB) * Ny];
+        for (int q = 0; q < P; q++) m[u][q] = gload<T>(src, voff, q * qstep);
     }
 #pragma unroll
     for (int u = 0; u < U; u++) {
@@ -260,9 +270,9 @@ __global__ __launch_bounds__(TPB* TR) PFC_WAVES(PFC_ROW_WAVES) void k_row_step(
             for (int q = 0; q < P; q++) w[q] = m[u][q];
             FFTF::run(w, L, twx, t);
             r2c_post<T, H, TPB, TR>(w, L, twx, t);
-            cpx<T>* dst = K + u * fstride + y;
+            const GBuf dst = gbuf(K + u * fstride);
 #pragma unroll
-            for (int q = 0; q < P; q++) dst[(size_t)(t + q * TPB) * Ny] = w[q];
+            for (int q = 0; q < P; q++) gstore<T>(dst, voff, q * qstep, w[q]);
         }
     }
 #pragma unroll
@@ -282,162 +292,165 @@ __global__ __launch_bounds__(TPB* TR) PFC_WAVES(PFC_ROW_WAVES) void k_row_step(
     for (int u = 0; u < U; u++) {
         FFTF::run(m[u], L, twx, t);
         r2c_post<T, H, TPB, TR>(m[u], L, twx, t);
-        cpx<T>* dst = J + u * fstride + y;
+        const GBuf dst = gbuf(J + u * fstride);
 #pragma unroll
-        for (int q = 0; q < P; q++) dst[(size_t)(t + q * TPB) * Ny] = m[u][q];
+        for (int q = 0; q < P; q++) gstore<T>(dst, voff, q * qstep, m[u][q]);
     }
 }
 
-// Unpacks the two Hermitian spectra of packed column 0 at ky index j:
-// P(j) = A(j) + i B(j); pj is P at the partner index (Ny - j) mod Ny.
+// Column addressing modes of the column passes.  mode 0: an ordinary column
+// kx of the row-spectral buffers; mode 1: the kx = 0 column, whose real input
+// is the .x half of packed column 0; mode 2: the kx = H column from the .y
+// half.  Modes 1 and 2 write back only the real parts of their inverse
+// transforms, into the same halves.
 template <typename T>
-__device__ __forceinline__ void unpack0(cpx<T> pv, cpx<T> pj, cpx<T>& a, cpx<T>& b) {
-    const cpx<T> c = cconj(pj);
-    a = cscale(cadd(pv, c), (T)0.5);
-    const cpx<T> d = csub(pv, c);      // 2i B
-    b = {(T)0.5 * d.y, (T)-0.5 * d.x};
+__device__ __forceinline__ cpx<T> col_in(GBuf b, unsigned voff, unsigned soff, int mode) {
+    const cpx<T> v = gload<T>(b, voff, soff);
+    if (mode == 0) return v;
+    return {mode == 1 ? v.x : v.y, (T)0};
+}
+template <typename T>
+__device__ __forceinline__ void col_out(GBuf b, unsigned voff, unsigned soff, int mode, cpx<T> v) {
+    if (mode == 0) gstore<T>(b, voff, soff, v);
+    else gstore_real<T>(b, voff + (mode == 2 ? (unsigned)sizeof(T) : 0u), soff, v.x);
 }
 
-// Column pass of a step for columns kx >= 1: TC columns per block (lane =
-// column-major within the block), each column all Ny rows.  Lanes of the
-// packed column 0 (block 0, c = 0) only take part in the barriers; that
-// column is handled by k_col0_step.
+// Which column lane group c of block b handles.  The grid has H/TC + 1
+// blocks: group 0 of block 0 takes kx = 0 and group 0 of the extra last
+// block kx = H (both from packed column 0, writing disjoint halves of it);
+// the other groups of the last block only join its barriers.
+struct ColJob {
+    int kx, mode;
+    bool live;
+};
+__device__ __forceinline__ ColJob col_job(int b, int c, int TC, int H) {
+    if (b == H / TC) return {H, 2, c == 0};
+    if (b == 0 && c == 0) return {0, 1, true};
+    return {b * TC + c, 0, true};
+}
+
+// Hermitian projection of the kx = 0 and kx = H columns (modes 1, 2) of a
+// block-0 pass, N(ky) <- (N(ky) + conj N(-ky))/2, through LDS.  The FP32 form
+// applies it to the state after every update: FP32 round-off in the
+// anti-Hermitian part would otherwise be amplified by the unstable modes
+// between refreshes and leak into the real parts.
+template <typename T, int NY, int TPB, int U>
+__device__ __forceinline__ void hermitian_cols(cpx<T> (&M)[U][NY / TPB], cpx<T>* L, int t, int mode) {
+    constexpr int P = NY / TPB;
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < P; q++) L[t + q * TPB] = M[u][q];
+        __syncthreads();
+        if (mode != 0) {
+#pragma unroll
+            for (int q = 0; q < P; q++) {
+                const cpx<T> pj = L[(NY - (t + q * TPB)) & (NY - 1)];
+                M[u][q] = cscale(cadd(M[u][q], cconj(pj)), (T)0.5);
+            }
+        }
+    }
+}
+
+// Column pass of a step: TC columns per block (lane = column-major within the
+// block), each column all Ny rows; the extra last block runs kx = H.
+// fstride: row-spectral field stride (H*Ny), nstride: state stride ((H+1)*Ny).
 template <typename T, int NY, int TPB, int TC, int U, bool REFRESH>
 __global__ __launch_bounds__(TPB* TC) PFC_WAVES(PFC_COL_WAVES) void k_col_step(
     const cpx<T>* __restrict__ J, const cpx<T>* __restrict__ K, cpx<T>* __restrict__ Nh,
-    cpx<T>* __restrict__ I, size_t fstride, const cpx<T>* __restrict__ twy,
-    const DevParams* __restrict__ prm) {
+    cpx<T>* __restrict__ I, cpx<T>* __restrict__ Nr, size_t fstride, size_t nstride, int H,
+    const cpx<T>* __restrict__ twy_, const DevParams* __restrict__ prm) {
     constexpr int P = NY / TPB;
+    constexpr unsigned E = sizeof(cpx<T>);
     using FFTF = BlockFFT<T, NY, TPB, -1, 1, 1>;
     using FFTI = BlockFFT<T, NY, TPB, +1, 1, 1>;
     __shared__ cpx<T> lds[NY * TC];
     const int c = threadIdx.x / TPB;
     const int t = threadIdx.x % TPB;
-    const int kx = blockIdx.x * TC + c;
-    const bool live = (kx != 0);
     cpx<T>* L = lds + c * NY;
-    const size_t col = (size_t)kx * NY;
+    const GBuf twy = gbuf(twy_);
+    {
+        const ColJob jb = col_job(blockIdx.x, c, TC, H);
+        const int kx = jb.kx;
+        // byte offsets of this lane's first element in a field's buffers
+        const unsigned vr = ((unsigned)(jb.mode ? 0 : kx) * NY + (unsigned)t) * E;
+        const unsigned vn = ((unsigned)kx * NY + (unsigned)t) * E;
 
-    cpx<T> M[U][P];
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-#pragma unroll
-        for (int q = 0; q < P; q++) M[u][q] = J[u * fstride + col + t + q * TPB];
-        FFTF::run(M[u], L, twy, t);
-    }
-    if constexpr (REFRESH) {
-        // N = FFT(n)/NxNy from the density's row transforms
-        const T s = (T)prm->inv_NxNy;
+        cpx<T> M[U][P];
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            cpx<T> w[P];
+            const GBuf src = gbuf(J + u * fstride);
 #pragma unroll
-            for (int q = 0; q < P; q++) w[q] = K[u * fstride + col + t + q * TPB];
-            FFTF::run(w, L, twy, t);
-            if (live) {
-#pragma unroll
-                for (int q = 0; q < P; q++) Nh[u * fstride + col + t + q * TPB] = cscale(w[q], s);
-            }
+            for (int q = 0; q < P; q++) M[u][q] = col_in<T>(src, vr, q * TPB * E, jb.mode);
+            FFTF::run(M[u], L, twy, t);
         }
-    }
-    if (live) {
-        const double dky = prm->dky;
-        const double kxd = kx * prm->dkx;
-        const double kx2d = kxd * kxd;
-#pragma unroll
-        for (int q = 0; q < P; q++) {
-            const int j = t + q * TPB;
-            const double ky = ky_of(j, NY, dky);
-            const T k2 = (T)(kx2d + ky * ky);
-            cpx<T> Nk[U];
-#pragma unroll
-            for (int u = 0; u < U; u++) Nk[u] = Nh[u * fstride + col + j];
-            cpx<T> Mk[U];
-#pragma unroll
-            for (int u = 0; u < U; u++) Mk[u] = M[u][q];
-            kupdate<U, T>(prm, k2, Mk, Nk);
-#pragma unroll
-            for (int u = 0; u < U; u++) {
-                Nh[u * fstride + col + j] = Nk[u];
-                M[u][q] = Nk[u];
-            }
-        }
-    }
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-        FFTI::run(M[u], L, twy, t);
-        if (live) {
-#pragma unroll
-            for (int q = 0; q < P; q++) I[u * fstride + col + t + q * TPB] = M[u][q];
-        }
-    }
-}
-
-// Column 0 of a step: the packed pair of Hermitian columns kx = 0 and
-// kx = H, P(ky) = N0(ky) + i NH(ky).  One block.  The forward transforms are
-// staged in the scratch S (2*U*Ny elements) so partners (Ny - ky) can be read.
-template <typename T, int NY, int TPB, int U, bool REFRESH>
-__global__ __launch_bounds__(TPB) void k_col0_step(const cpx<T>* __restrict__ J,
-                                                   const cpx<T>* __restrict__ K,
-                                                   cpx<T>* __restrict__ Nh,
-                                                   cpx<T>* __restrict__ I, cpx<T>* __restrict__ S,
-                                                   size_t fstride, int H,
-                                                   const cpx<T>* __restrict__ twy,
-                                                   const DevParams* __restrict__ prm) {
-    constexpr int P = NY / TPB;
-    using FFTF = BlockFFT<T, NY, TPB, -1, 1, 1>;
-    using FFTI = BlockFFT<T, NY, TPB, +1, 1, 1>;
-    __shared__ cpx<T> lds[NY];
-    const int t = threadIdx.x;
-    cpx<T>* Ms = S;             // packed M of every field
-    cpx<T>* Ps = S + U * NY;    // packed new state of every field
-    for (int u = 0; u < U; u++) {
-        cpx<T> w[P];
-#pragma unroll
-        for (int q = 0; q < P; q++) w[q] = J[u * fstride + t + q * TPB];
-        FFTF::run(w, lds, twy, t);
-#pragma unroll
-        for (int q = 0; q < P; q++) Ms[u * NY + t + q * TPB] = w[q];
+        // state read by the update: Nh, or on refresh steps N = FFT(n)/NxNy,
+        // computed here from the density's row transforms into the scratch Nr
+        // (so that no address of Nh is written twice by this kernel)
+        cpx<T>* const Nin = REFRESH ? Nr : Nh;
         if constexpr (REFRESH) {
-#pragma unroll
-            for (int q = 0; q < P; q++) w[q] = K[u * fstride + t + q * TPB];
-            FFTF::run(w, lds, twy, t);
             const T s = (T)prm->inv_NxNy;
 #pragma unroll
-            for (int q = 0; q < P; q++) Nh[u * fstride + t + q * TPB] = cscale(w[q], s);
-        }
-    }
-    __syncthreads();
-    const double dky = prm->dky;
-    const double kxh = (double)H * prm->dkx;
-    const double kxh2 = kxh * kxh;
+            for (int u = 0; u < U; u++) {
+                const GBuf src = gbuf(K + u * fstride);
+                const GBuf dst = gbuf(Nr + u * nstride);
+                cpx<T> w[P];
 #pragma unroll
-    for (int q = 0; q < P; q++) {
-        const int j = t + q * TPB;
-        const int jp = (NY - j) & (NY - 1);
-        const double ky = ky_of(j, NY, dky);
-        cpx<T> M0[U], MH[U], N0[U], NH[U];
+                for (int q = 0; q < P; q++) w[q] = col_in<T>(src, vr, q * TPB * E, jb.mode);
+                FFTF::run(w, L, twy, t);
+                if (jb.live) {
+#pragma unroll
+                    for (int q = 0; q < P; q++) gstore<T>(dst, vn, q * TPB * E, cscale(w[q], s));
+                }
+            }
+        }
+        if (jb.live) {
+            const double dky = prm->dky;
+            const double kxd = kx * prm->dkx;
+            const double kx2d = kxd * kxd;
+#pragma unroll
+            for (int q = 0; q < P; q++) {
+                const int j = t + q * TPB;
+                const double ky = ky_of(j, NY, dky);
+                const T k2 = (T)(kx2d + ky * ky);
+                cpx<T> Mk[U], Nk[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    Mk[u] = M[u][q];
+                    Nk[u] = gload<T>(gbuf(Nin + u * nstride), vn, q * TPB * E);
+                }
+                kupdate<U, T>(prm, k2, Mk, Nk);
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    gstore<T>(gbuf(Nh + u * nstride), vn, q * TPB * E, Nk[u]);
+                    M[u][q] = Nk[u];
+                }
+            }
+        }
+        if constexpr (sizeof(T) == 4) {
+            // kx = 0 and kx = H: replace the stored state by its Hermitian part
+            if (blockIdx.x == 0 || blockIdx.x == H / TC) {
+                hermitian_cols<T, NY, TPB, U>(M, L, t, jb.live ? jb.mode : 0);
+                if (jb.live && jb.mode != 0) {
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        const GBuf nb = gbuf(Nh + u * nstride);
+#pragma unroll
+                        for (int q = 0; q < P; q++) gstore<T>(nb, vn, q * TPB * E, M[u][q]);
+                    }
+                }
+            }
+        }
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            unpack0(Ms[u * NY + j], Ms[u * NY + jp], M0[u], MH[u]);
-            unpack0(Nh[u * fstride + j], Nh[u * fstride + jp], N0[u], NH[u]);
+            FFTI::run(M[u], L, twy, t);
+            if (jb.live) {
+                const GBuf dst = gbuf(I + u * fstride);
+#pragma unroll
+                for (int q = 0; q < P; q++) col_out<T>(dst, vr, q * TPB * E, jb.mode, M[u][q]);
+            }
         }
-        kupdate<U, T>(prm, (T)(ky * ky), M0, N0);
-        kupdate<U, T>(prm, (T)(kxh2 + ky * ky), MH, NH);
-#pragma unroll
-        for (int u = 0; u < U; u++) Ps[u * NY + j] = {N0[u].x - NH[u].y, N0[u].y + NH[u].x};
-    }
-    __syncthreads();   // all reads of the old state are done
-    for (int u = 0; u < U; u++) {
-        cpx<T> w[P];
-#pragma unroll
-        for (int q = 0; q < P; q++) {
-            w[q] = Ps[u * NY + t + q * TPB];
-            Nh[u * fstride + t + q * TPB] = w[q];
-        }
-        FFTI::run(w, lds, twy, t);
-#pragma unroll
-        for (int q = 0; q < P; q++) I[u * fstride + t + q * TPB] = w[q];
     }
 }
 
@@ -450,7 +463,8 @@ template <typename T, int NX, int TPB, int TR>
 __global__ __launch_bounds__(TPB* TR) void k_row_c2r(const cpx<T>* __restrict__ src,
                                                      T* __restrict__ dst, cpx<T>* __restrict__ K,
                                                      size_t fstride, size_t rstride, int Ny,
-                                                     const cpx<T>* __restrict__ twx) {
+                                                     const cpx<T>* __restrict__ twx_) {
+    const GBuf twx = gbuf(twx_);
     constexpr int H = NX / 2;
     constexpr int P = H / TPB;
     __shared__ cpx<T> lds[H * TR];
@@ -460,21 +474,25 @@ __global__ __launch_bounds__(TPB* TR) void k_row_c2r(const cpx<T>* __restrict__ 
     const int y = xcd_remap(blockIdx.x, gridDim.x) * TR + r;
     const int b = blockIdx.y;
     cpx<T>* L = lds + r;
+    constexpr unsigned E = sizeof(cpx<T>);
+    const unsigned voff = ((unsigned)t * (unsigned)Ny + (unsigned)y) * E;
+    const unsigned qstep = (unsigned)TPB * (unsigned)Ny * E;
     cpx<T> v[P];
-    const cpx<T>* s = src + b * fstride + y;
+    const GBuf sb = gbuf(src + b * fstride);
 #pragma unroll
-    for (int q = 0; q < P; q++) v[q] = s[(size_t)(t + q * TPB) * Ny];
+    for (int q = 0; q < P; q++) v[q] = gload<T>(sb, voff, q * qstep);
     c2r_pre<T, H, TPB, TR>(v, L, twx, t);
     BlockFFT<T, H, TPB, +1, 2, TR>::run(v, L, twx, t);
-    cpx<T>* d = reinterpret_cast<cpx<T>*>(dst + b * rstride + (size_t)y * NX);
+    const GBuf db = gbuf(dst + b * rstride);
+    const unsigned roff = ((unsigned)y * NX / 2 + (unsigned)t) * E;
 #pragma unroll
-    for (int q = 0; q < P; q++) d[t + q * TPB] = v[q];
+    for (int q = 0; q < P; q++) gstore<T>(db, roff, q * TPB * E, v[q]);
     if (K) {
         BlockFFT<T, H, TPB, -1, 2, TR>::run(v, L, twx, t);
         r2c_post<T, H, TPB, TR>(v, L, twx, t);
-        cpx<T>* k = K + b * fstride + y;
+        const GBuf kb = gbuf(K + b * fstride);
 #pragma unroll
-        for (int q = 0; q < P; q++) k[(size_t)(t + q * TPB) * Ny] = v[q];
+        for (int q = 0; q < P; q++) gstore<T>(kb, voff, q * qstep, v[q]);
     }
 }
 
@@ -484,7 +502,8 @@ template <typename T, int NX, int TPB, int TR>
 __global__ __launch_bounds__(TPB* TR) void k_row_r2c(const T* __restrict__ src,
                                                      cpx<T>* __restrict__ dst, size_t fstride,
                                                      size_t rstride, int Ny,
-                                                     const cpx<T>* __restrict__ twx, T scale) {
+                                                     const cpx<T>* __restrict__ twx_, T scale) {
+    const GBuf twx = gbuf(twx_);
     constexpr int H = NX / 2;
     constexpr int P = H / TPB;
     __shared__ cpx<T> lds[H * TR];
@@ -494,36 +513,52 @@ __global__ __launch_bounds__(TPB* TR) void k_row_r2c(const T* __restrict__ src,
     const int y = xcd_remap(blockIdx.x, gridDim.x) * TR + r;
     const int b = blockIdx.y;
     cpx<T>* L = lds + r;
+    constexpr unsigned E = sizeof(cpx<T>);
     cpx<T> v[P];
-    const cpx<T>* s = reinterpret_cast<const cpx<T>*>(src + b * rstride + (size_t)y * NX);
+    const GBuf sb = gbuf(src + b * rstride);
+    const unsigned roff = ((unsigned)y * NX / 2 + (unsigned)t) * E;
 #pragma unroll
-    for (int q = 0; q < P; q++) v[q] = s[t + q * TPB];
+    for (int q = 0; q < P; q++) v[q] = gload<T>(sb, roff, q * TPB * E);
     BlockFFT<T, H, TPB, -1, 2, TR>::run(v, L, twx, t);
     r2c_post<T, H, TPB, TR>(v, L, twx, t);
-    cpx<T>* d = dst + b * fstride + y;
+    const GBuf db = gbuf(dst + b * fstride);
+    const unsigned voff = ((unsigned)t * (unsigned)Ny + (unsigned)y) * E;
+    const unsigned qstep = (unsigned)TPB * (unsigned)Ny * E;
 #pragma unroll
-    for (int q = 0; q < P; q++) d[(size_t)(t + q * TPB) * Ny] = cscale(v[q], scale);
+    for (int q = 0; q < P; q++) gstore<T>(db, voff, q * qstep, cscale(v[q], scale));
 }
 
-// Column forward FFT of nbuf buffers with a scale: dst = scale * FFT_y(src).
+// Column forward FFT of nbuf buffers with a scale, from packed row-spectral
+// buffers (H columns) into unpacked state buffers (H+1 columns):
+// dst = scale * FFT_y(src).
 template <typename T, int NY, int TPB, int TC>
 __global__ __launch_bounds__(TPB* TC) void k_col_fwd(const cpx<T>* __restrict__ src,
                                                      cpx<T>* __restrict__ dst, size_t fstride,
-                                                     const cpx<T>* __restrict__ twy, double scale) {
+                                                     size_t nstride, int H,
+                                                     const cpx<T>* __restrict__ twy_,
+                                                     double scale) {
+    const GBuf twy = gbuf(twy_);
     constexpr int P = NY / TPB;
+    constexpr unsigned E = sizeof(cpx<T>);
     __shared__ cpx<T> lds[NY * TC];
     const int c = threadIdx.x / TPB;
     const int t = threadIdx.x % TPB;
-    const int kx = blockIdx.x * TC + c;
     const int b = blockIdx.y;
     cpx<T>* L = lds + c * NY;
-    const size_t col = b * fstride + (size_t)kx * NY;
-    cpx<T> v[P];
+    const GBuf sb = gbuf(src + b * fstride), db = gbuf(dst + b * nstride);
+    {
+        const ColJob jb = col_job(blockIdx.x, c, TC, H);
+        const unsigned vr = ((unsigned)(jb.mode ? 0 : jb.kx) * NY + (unsigned)t) * E;
+        const unsigned vn = ((unsigned)jb.kx * NY + (unsigned)t) * E;
+        cpx<T> v[P];
 #pragma unroll
-    for (int q = 0; q < P; q++) v[q] = src[col + t + q * TPB];
-    BlockFFT<T, NY, TPB, -1, 1, 1>::run(v, L, twy, t);
+        for (int q = 0; q < P; q++) v[q] = col_in<T>(sb, vr, q * TPB * E, jb.mode);
+        BlockFFT<T, NY, TPB, -1, 1, 1>::run(v, L, twy, t);
+        if (jb.live) {
 #pragma unroll
-    for (int q = 0; q < P; q++) dst[col + t + q * TPB] = cscale(v[q], (T)scale);
+            for (int q = 0; q < P; q++) gstore<T>(db, vn, q * TPB * E, cscale(v[q], (T)scale));
+        }
+    }
 }
 
 // Multiplier of one evaluate output at k2 (src/pfc-relax.c:1048-1077):
@@ -549,81 +584,69 @@ __device__ __forceinline__ double eval_mult(const DevParams* __restrict__ p, int
 // Evaluate's k-space pass for source field s = blockIdx.y: M = FFT_y(K_s)/NxNy
 // (also stored as the refreshed state Nh_s, src/pfc-relax.c:1124-1128), then
 // every output that multiplies M_s -- P_s M_s, G M_s and (c2_us - k2)^2 M_s
-// for u < s -- is transformed back along y into O.
+// for u < s -- is transformed back along y into O (packed like K).
 // O buffer order: [P_0..P_{U-1}, G_0..G_{U-1}, q_uv for u<v in row order].
 template <typename T, int NY, int TPB, int TC>
 __global__ __launch_bounds__(TPB* TC) void k_col_eval(const cpx<T>* __restrict__ K,
                                                       cpx<T>* __restrict__ Nh,
                                                       cpx<T>* __restrict__ O, size_t fstride,
-                                                      int H, const cpx<T>* __restrict__ twy,
+                                                      size_t nstride, int H,
+                                                      const cpx<T>* __restrict__ twy_,
                                                       const DevParams* __restrict__ prm) {
+    const GBuf twy = gbuf(twy_);
     constexpr int P = NY / TPB;
+    constexpr unsigned E = sizeof(cpx<T>);
     __shared__ cpx<T> lds[NY * TC];
     const int c = threadIdx.x / TPB;
     const int t = threadIdx.x % TPB;
-    const int kx = blockIdx.x * TC + c;
     const int s = blockIdx.y;
     const int U = prm->U;
     cpx<T>* L = lds + c * NY;
-    const size_t col = (size_t)kx * NY;
     const T scale = (T)prm->inv_NxNy;
-    cpx<T> M[P];
+    const GBuf kb = gbuf(K + s * fstride), nb = gbuf(Nh + s * nstride);
+    {
+        const ColJob jb = col_job(blockIdx.x, c, TC, H);
+        const unsigned vr = ((unsigned)(jb.mode ? 0 : jb.kx) * NY + (unsigned)t) * E;
+        const unsigned vn = ((unsigned)jb.kx * NY + (unsigned)t) * E;
+        cpx<T> M[P];
 #pragma unroll
-    for (int q = 0; q < P; q++) M[q] = K[s * fstride + col + t + q * TPB];
-    BlockFFT<T, NY, TPB, -1, 1, 1>::run(M, L, twy, t);
-#pragma unroll
-    for (int q = 0; q < P; q++) {
-        M[q] = cscale(M[q], scale);
-        Nh[s * fstride + col + t + q * TPB] = M[q];
-    }
-    // partner values for the packed column 0 (block 0 only; all lanes take part)
-    cpx<T> Mp[P];
-    if (blockIdx.x == 0) {
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < P; q++) L[t + q * TPB] = M[q];
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < P; q++) Mp[q] = L[(NY - (t + q * TPB)) & (NY - 1)];
-    }
-    const double dky = prm->dky;
-    const double kxd = kx * prm->dkx;
-    const double kx2 = kxd * kxd;
-    const double kxh = (double)H * prm->dkx;
-    const double kxh2 = kxh * kxh;
-    // outputs with source s: (kind, field index for the multiplier, c2, out index)
-    const int nout = 2 + s;
-    for (int o = 0; o < nout; o++) {
-        int kind, fu, oi;
-        double c2 = 0.0;
-        if (o == 0) { kind = 0; fu = s; oi = s; }
-        else if (o == 1) { kind = 1; fu = s; oi = U + s; }
-        else {
-            const int u = o - 2;   // pair (u, s), u < s
-            kind = 2; fu = u;
-            c2 = prm->qc2[u][s];
-            oi = 2 * U + u * U - u * (u + 1) / 2 + (s - u - 1);
-        }
-        cpx<T> X[P];
+        for (int q = 0; q < P; q++) M[q] = col_in<T>(kb, vr, q * TPB * E, jb.mode);
+        BlockFFT<T, NY, TPB, -1, 1, 1>::run(M, L, twy, t);
 #pragma unroll
         for (int q = 0; q < P; q++) {
-            const int j = t + q * TPB;
-            const double ky = ky_of(j, NY, dky);
-            if (kx == 0) {
-                cpx<T> a, b;
-                unpack0(M[q], Mp[q], a, b);
-                const T f0 = (T)eval_mult(prm, kind, fu, c2, ky * ky);
-                const T fh = (T)eval_mult(prm, kind, fu, c2, kxh2 + ky * ky);
-                a = cscale(a, f0);
-                b = cscale(b, fh);
-                X[q] = {a.x - b.y, a.y + b.x};
-            } else {
+            M[q] = cscale(M[q], scale);
+            if (jb.live) gstore<T>(nb, vn, q * TPB * E, M[q]);
+        }
+        const double dky = prm->dky;
+        const double kxd = jb.kx * prm->dkx;
+        const double kx2 = kxd * kxd;
+        // outputs with source s: (kind, field index for the multiplier, c2, out index)
+        const int nout = 2 + s;
+        for (int o = 0; o < nout; o++) {
+            int kind, fu, oi;
+            double c2 = 0.0;
+            if (o == 0) { kind = 0; fu = s; oi = s; }
+            else if (o == 1) { kind = 1; fu = s; oi = U + s; }
+            else {
+                const int u = o - 2;   // pair (u, s), u < s
+                kind = 2; fu = u;
+                c2 = prm->qc2[u][s];
+                oi = 2 * U + u * U - u * (u + 1) / 2 + (s - u - 1);
+            }
+            cpx<T> X[P];
+#pragma unroll
+            for (int q = 0; q < P; q++) {
+                const int j = t + q * TPB;
+                const double ky = ky_of(j, NY, dky);
                 X[q] = cscale(M[q], (T)eval_mult(prm, kind, fu, c2, kx2 + ky * ky));
             }
-        }
-        BlockFFT<T, NY, TPB, +1, 1, 1>::run(X, L, twy, t);
+            BlockFFT<T, NY, TPB, +1, 1, 1>::run(X, L, twy, t);
+            const GBuf ob = gbuf(O + oi * fstride);
+            if (jb.live) {
 #pragma unroll
-        for (int q = 0; q < P; q++) O[oi * fstride + col + t + q * TPB] = X[q];
+                for (int q = 0; q < P; q++) col_out<T>(ob, vr, q * TPB * E, jb.mode, X[q]);
+            }
+        }
     }
 }
 

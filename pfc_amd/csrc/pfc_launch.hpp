@@ -15,17 +15,22 @@ constexpr int kEnergyBS = 256;
 struct RowCfg {
     int tpb, tr;
 };
-// row transforms have length H = Nx/2; TR rows share a block of TPB*TR lanes
+// Row transforms have length H = Nx/2 and TR rows share a block of TPB*TR
+// lanes.  TR is as large as the on-chip budget of a block allows (all U
+// fields of TR rows stay in registers), so that TR*sizeof(E) bytes of each
+// column-major buffer are read and written contiguously.
+template <typename T>
 constexpr RowCfg row_cfg(int nx) {
+    constexpr bool f64 = sizeof(T) == 8;
     switch (nx) {
         case 64: return {8, 32};
         case 128: return {8, 32};
         case 256: return {16, 16};
         case 512: return {32, 8};
         case 1024: return {64, 4};
-        case 2048: return {128, 2};
-        case 4096: return {256, 1};
-        case 8192: return {512, 1};
+        case 2048: return {128, f64 ? 4 : 8};
+        case 4096: return {256, f64 ? 2 : 4};
+        case 8192: return {512, f64 ? 1 : 2};
         default: return {0, 0};
     }
 }
@@ -46,14 +51,16 @@ constexpr ColCfg col_cfg(int ny) {
     }
 }
 
-inline bool size_ok(int n) { return row_cfg(n).tpb != 0; }
+inline bool size_ok(int n) { return row_cfg<double>(n).tpb != 0; }
 
 }  // namespace pfc
 
 // Device context behind the opaque pfc_ctx of the C ABI.
 struct pfc_ctx {
     int Nx, Ny, U, prec, nO;
-    size_t nk, nr;
+    size_t nk;    // row-spectral buffer elements per field, (Nx/2) * Ny
+    size_t nkn;   // spectral-state elements per field, (Nx/2 + 1) * Ny
+    size_t nr;    // real-space points, Nx * Ny
     size_t esz;  // bytes per complex element
     pfc_params p;
     double dx, dy;
@@ -64,7 +71,7 @@ struct pfc_ctx {
     int* nan_flag;
     hipStream_t stream;
     int have_fields, have_box, have_f;
-    // pass timing (pfc_timing): 4 events per recorded step
+    // pass timing (pfc_timing): 3 events per recorded step
     int timing;
     int nrec;
     int nev;
@@ -81,7 +88,7 @@ struct Launch {
 
     template <int NX, int U, bool REF>
     static void row_step(pfc_ctx* c, int t) {
-        constexpr RowCfg g = row_cfg(NX);
+        constexpr RowCfg g = row_cfg<T>(NX);
         hipLaunchKernelGGL((k_row_step<T, NX, g.tpb, g.tr, U, REF>), dim3(c->Ny / g.tr),
                            dim3(g.tpb * g.tr), 0, c->stream, (const C*)c->I, (C*)c->J, (C*)c->K,
                            c->nk, c->Ny, (const C*)c->twx, c->dprm, c->nan_flag, t);
@@ -114,47 +121,44 @@ struct Launch {
     }
 
     template <int NY, int U, bool REF>
-    static void col_step(pfc_ctx* c, hipEvent_t mid) {
+    static void col_step(pfc_ctx* c) {
         constexpr ColCfg g = col_cfg(NY);
         const int H = c->Nx / 2;
-        hipLaunchKernelGGL((k_col0_step<T, NY, g.tpb, U, REF>), dim3(1), dim3(g.tpb), 0,
-                           c->stream, (const C*)c->J, (const C*)c->K, (C*)c->Nh, (C*)c->I,
-                           (C*)c->O, c->nk, H, (const C*)c->twy, c->dprm);
-        if (mid) (void)hipEventRecord(mid, c->stream);
-        hipLaunchKernelGGL((k_col_step<T, NY, g.tpb, g.tc, U, REF>), dim3(H / g.tc),
+        hipLaunchKernelGGL((k_col_step<T, NY, g.tpb, g.tc, U, REF>), dim3(H / g.tc + 1),
                            dim3(g.tpb * g.tc), 0, c->stream, (const C*)c->J, (const C*)c->K,
-                           (C*)c->Nh, (C*)c->I, c->nk, (const C*)c->twy, c->dprm);
+                           (C*)c->Nh, (C*)c->I, (C*)c->O, c->nk, c->nkn, H, (const C*)c->twy,
+                           c->dprm);
     }
     template <int NY, int U>
-    static void col_step_u(pfc_ctx* c, bool ref, hipEvent_t mid) {
-        if (ref) col_step<NY, U, true>(c, mid);
-        else col_step<NY, U, false>(c, mid);
+    static void col_step_u(pfc_ctx* c, bool ref) {
+        if (ref) col_step<NY, U, true>(c);
+        else col_step<NY, U, false>(c);
     }
     template <int NY>
-    static void col_step_n(pfc_ctx* c, bool ref, hipEvent_t mid) {
+    static void col_step_n(pfc_ctx* c, bool ref) {
         switch (c->U) {
-            case 1: col_step_u<NY, 1>(c, ref, mid); break;
-            case 2: col_step_u<NY, 2>(c, ref, mid); break;
-            case 3: col_step_u<NY, 3>(c, ref, mid); break;
-            case 4: col_step_u<NY, 4>(c, ref, mid); break;
+            case 1: col_step_u<NY, 1>(c, ref); break;
+            case 2: col_step_u<NY, 2>(c, ref); break;
+            case 3: col_step_u<NY, 3>(c, ref); break;
+            case 4: col_step_u<NY, 4>(c, ref); break;
         }
     }
-    static void col_step_any(pfc_ctx* c, bool ref, hipEvent_t mid) {
+    static void col_step_any(pfc_ctx* c, bool ref) {
         switch (c->Ny) {
-            case 64: col_step_n<64>(c, ref, mid); break;
-            case 128: col_step_n<128>(c, ref, mid); break;
-            case 256: col_step_n<256>(c, ref, mid); break;
-            case 512: col_step_n<512>(c, ref, mid); break;
-            case 1024: col_step_n<1024>(c, ref, mid); break;
-            case 2048: col_step_n<2048>(c, ref, mid); break;
-            case 4096: col_step_n<4096>(c, ref, mid); break;
-            case 8192: col_step_n<8192>(c, ref, mid); break;
+            case 64: col_step_n<64>(c, ref); break;
+            case 128: col_step_n<128>(c, ref); break;
+            case 256: col_step_n<256>(c, ref); break;
+            case 512: col_step_n<512>(c, ref); break;
+            case 1024: col_step_n<1024>(c, ref); break;
+            case 2048: col_step_n<2048>(c, ref); break;
+            case 4096: col_step_n<4096>(c, ref); break;
+            case 8192: col_step_n<8192>(c, ref); break;
         }
     }
 
     template <int NX>
     static void row_c2r(pfc_ctx* c, const void* src, void* dst, void* K, int nbuf) {
-        constexpr RowCfg g = row_cfg(NX);
+        constexpr RowCfg g = row_cfg<T>(NX);
         hipLaunchKernelGGL((k_row_c2r<T, NX, g.tpb, g.tr>), dim3(c->Ny / g.tr, nbuf),
                            dim3(g.tpb * g.tr), 0, c->stream, (const C*)src, (T*)dst, (C*)K, c->nk,
                            c->nr, c->Ny, (const C*)c->twx);
@@ -173,7 +177,7 @@ struct Launch {
     }
     template <int NX>
     static void row_r2c(pfc_ctx* c, const void* src, void* dst, int nbuf) {
-        constexpr RowCfg g = row_cfg(NX);
+        constexpr RowCfg g = row_cfg<T>(NX);
         hipLaunchKernelGGL((k_row_r2c<T, NX, g.tpb, g.tr>), dim3(c->Ny / g.tr, nbuf),
                            dim3(g.tpb * g.tr), 0, c->stream, (const T*)src, (C*)dst, c->nk, c->nr,
                            c->Ny, (const C*)c->twx, (T)(1.0 / NX));
@@ -194,9 +198,9 @@ struct Launch {
     static void col_fwd(pfc_ctx* c, const void* src, void* dst, int nbuf, double scale) {
         constexpr ColCfg g = col_cfg(NY);
         const int H = c->Nx / 2;
-        hipLaunchKernelGGL((k_col_fwd<T, NY, g.tpb, g.tc>), dim3(H / g.tc, nbuf),
+        hipLaunchKernelGGL((k_col_fwd<T, NY, g.tpb, g.tc>), dim3(H / g.tc + 1, nbuf),
                            dim3(g.tpb * g.tc), 0, c->stream, (const C*)src, (C*)dst, c->nk,
-                           (const C*)c->twy, scale);
+                           c->nkn, H, (const C*)c->twy, scale);
     }
     static void col_fwd_any(pfc_ctx* c, const void* src, void* dst, int nbuf, double scale) {
         switch (c->Ny) {
@@ -214,9 +218,9 @@ struct Launch {
     static void col_eval(pfc_ctx* c) {
         constexpr ColCfg g = col_cfg(NY);
         const int H = c->Nx / 2;
-        hipLaunchKernelGGL((k_col_eval<T, NY, g.tpb, g.tc>), dim3(H / g.tc, c->U),
+        hipLaunchKernelGGL((k_col_eval<T, NY, g.tpb, g.tc>), dim3(H / g.tc + 1, c->U),
                            dim3(g.tpb * g.tc), 0, c->stream, (const C*)c->K, (C*)c->Nh, (C*)c->O,
-                           c->nk, H, (const C*)c->twy, c->dprm);
+                           c->nk, c->nkn, H, (const C*)c->twy, c->dprm);
     }
     static void col_eval_any(pfc_ctx* c) {
         switch (c->Ny) {

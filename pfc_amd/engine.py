@@ -14,7 +14,7 @@ import numpy as np
 from .params import FP64, MIXED, PfcParams
 
 PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG, "lib", "libpfcgpu.so")
+LIB_PATH = os.environ.get("PFC_LIB") or os.path.join(PKG, "lib", "libpfcgpu.so")
 BIN_PATH = os.path.join(PKG, "bin", "pfc-relax")
 
 # exported symbols of include/pfc_relax.h
@@ -139,8 +139,8 @@ class Engine:
         self._check(load().pfc_timing(self._h, 1 if enable else 0), "pfc_timing")
 
     def timing_read(self):
-        """Summed (row, column-0, column) pass milliseconds and step count."""
-        ms = np.zeros(3)
+        """Summed (row, column) pass milliseconds and step count."""
+        ms = np.zeros(2)
         n = ctypes.c_int()
         self._check(load().pfc_timing_read(self._h, _dp(ms), ctypes.byref(n)), "pfc_timing_read")
         return ms, n.value

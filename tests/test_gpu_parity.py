@@ -60,6 +60,17 @@ def test_roundtrip_and_one_step_sizes(gpu, Nx, Ny):
         assert rel_l2(g.get_fields(), o.get_fields()) < 1e-13
 
 
+@pytest.mark.parametrize("precision", [FP64, MIXED])
+@pytest.mark.parametrize("Nx,Ny", [(4096, 64), (64, 4096), (8192, 64), (64, 8192), (2048, 128)])
+def test_wide_and_tall_grids(gpu, Nx, Ny, precision):
+    """Every row/column kernel configuration of the large sizes, 3 fields."""
+    p = hetero_params()
+    f = hetero_patchwork(Nx, Ny, 0.7, 0.7, domain=48, seed=11)
+    o, g = run_pair(p, f, Nx, Ny, 0.7, 0.7, 0, 3, precision=precision)
+    err = rel_l2(g.get_fields(), o.get_fields())
+    assert err < (1e-13 if precision == FP64 else 2e-6), err
+
+
 @pytest.mark.parametrize("c", [0.0, 1.0])
 def test_graphene_steps_fp64(gpu, c):
     Nx, Ny = 128, 128
