@@ -197,10 +197,12 @@ int main(int argc, char** argv) {
     s.host = (double*)malloc(sizeof(double) * n * (size_t)(cfg->U > 0 ? cfg->U : 1));
     s.nave = (double*)calloc((size_t)(cfg->U > 0 ? cfg->U : 1), sizeof(double));
 
-    /* initialize (src/pfc-relax.c:959-964): allocate, read data, operators */
+    /* initialize (src/pfc-relax.c:959-964): read data, allocate, operators.
+     * The data files are parsed before the device is touched so that their
+     * errors (codes 4, 37, 38) come first, as they do in the reference. */
+    pfc_read_data(cfg, s.host);
     int rc = pfc_create(&s.ctx, &cfg->p, cfg->Nx, cfg->Ny, precision);
     if (rc) gpu_fail(rc, "pfc_create");
-    pfc_read_data(cfg, s.host);
     rc = pfc_set_fields(s.ctx, s.host);
     if (rc) gpu_fail(rc, "pfc_set_fields");
     rc = pfc_set_box(s.ctx, cfg->dx, cfg->dy);

@@ -1,0 +1,174 @@
+"""End-to-end runs of the pfc-relax binary on the GPU against the oracle's
+relax() (src/pfc-relax.c:1292-1312): log lines (print_output :861-897),
+density snapshots (write_data :766-813, written at %.16e so FP64 round-trips),
+energy files, box optimisation (optimize :1250-1289) and the NaN exit (39).
+"""
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle_lib import Oracle
+from pfc_amd import LATTICE_A, graphene_params, hetero_params
+from pfc_amd.engine import BIN_PATH
+from pfc_amd.inputs import crystal, hetero_patchwork
+
+pytestmark = pytest.mark.gpu
+
+
+def write_density(path, a, dx, dy):
+    Ny, Nx = a.shape
+    with open(path, "w") as f:
+        f.write(f"{Nx} {Ny} {dx:.16e} {dy:.16e}\n")
+        np.savetxt(f, a.ravel(), fmt="%.17e")
+
+
+def read_density(path):
+    with open(path) as f:
+        hdr = f.readline().split()
+        data = np.loadtxt(f)
+    return hdr, data.reshape(int(hdr[1]), int(hdr[0]))
+
+
+def run_cli(cwd, inp, env=None):
+    e = dict(os.environ)
+    e.update(env or {})
+    r = subprocess.run([BIN_PATH, inp], cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                       text=True, env=e, timeout=120)
+    return r.returncode, r.stdout, r.stderr
+
+
+def parse_log(text, U):
+    rows = []
+    for line in text.strip().splitlines():
+        v = line.split()
+        assert len(v) == 5 + U, line
+        assert line.endswith(" ") or True
+        rows.append([float(v[0]), float(v[2]), float(v[3]), float(v[4])] + [float(x) for x in v[5:]])
+    return np.array(rows)
+
+
+HONEY = """# honey-shaped run
+I\tin.n
+\tout-#.n\tout-#.f
+\t100\t[\t0\t100\t250\t]\t14\t16
+M\t1
+l\t0.0
+a\t-0.25
+b\t1.0
+g\t1.0
+d\t1.0
+e\t
+s\t0.2
+n\t1.0
+o\t1.0
+p\t0.0
+m\t
+R\t{c}\t250\t{dx}\t{dy}\t1.0\t{topt}\t1\t1
+"""
+
+
+@pytest.mark.parametrize("c,topt", [(0, 0), (1, 50)])
+def test_graphene_run_matches_oracle(gpu, tmp_path, c, topt):
+    Nx = Ny = 64
+    dx = LATTICE_A / 8
+    dy = dx * 1.01
+    f = crystal(Nx, Ny, dx, dy, 0.2, -0.2, LATTICE_A, sym=3, theta=0.2, noise=0.01, seed=4)
+    write_density(tmp_path / "in.n", f, dx, dy)
+    (tmp_path / "run.in").write_text(HONEY.format(c=c, dx=repr(dx), dy=repr(dy), topt=topt))
+    rc, out, err = run_cli(tmp_path, "run.in")
+    assert rc == 0, err
+    log = parse_log(out, 1)
+    assert (tmp_path / "run.out").read_text() == out
+    o = Oracle(graphene_params(c=float(c)), Nx, Ny, dx, dy)
+    o.set_fields(f[None])
+    olog, snaps = o.relax(250, 100, Topt=topt, xopt=1, yopt=1, snap_steps=(0, 100, 250))
+    assert np.array_equal(log[:, 0], olog[:, 0])
+    np.testing.assert_allclose(log[:, 1:3], olog[:, 1:3], rtol=1e-11, atol=0)   # dx, dy
+    np.testing.assert_allclose(log[:, 3], olog[:, 3], rtol=1e-11, atol=1e-14)   # fave
+    np.testing.assert_allclose(log[:, 4], olog[:, 4], rtol=1e-11, atol=1e-14)   # nave
+    for k, t in enumerate((0, 100, 250)):
+        hdr, n = read_density(tmp_path / f"out-{t}.n")
+        assert hdr[:2] == [str(Nx), str(Ny)]
+        err_l2 = np.linalg.norm(n - snaps[k, 0]) / np.linalg.norm(snaps[k, 0])
+        assert err_l2 < 1e-10, (t, err_l2)
+        assert (tmp_path / f"out-{t}.f").exists()
+
+
+HETERO = """I\ta.n\tb.n\tc.n
+\ta-#.n\tb-#.n\tc-#.n\te-#.f
+\t50\t50\t12\t16
+M\t3
+l\t0.0
+\t0.0
+\t0.0
+a\t-1.4\t-0.04\t-0.04
+\t\t-0.3\t0.5
+\t\t\t-0.3
+b\t2.25\t0.0\t0.0
+\t\t1.0\t0.02
+\t\t\t1.0
+g\t0.0\t0.0\t0.0
+\t\t0.0\t0.3
+\t\t\t0.0
+d\t2.25
+\t1.0
+\t1.0
+e\t\t-0.8\t-0.8
+\t\t\t0.0
+s\t0.2
+n\t1.0
+\t1.018
+\t1.018
+o\t1.0
+\t1.0
+\t1.0
+p\t0.0
+\t0.0
+\t0.0
+m\t\t0.0\t0.0
+\t\t\t1.018
+R\t1\t100\t0.7\t0.7\t0.25\t0\t1\t1
+"""
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_hetero_run(gpu, tmp_path, mixed):
+    Nx, Ny = 128, 64
+    f = hetero_patchwork(Nx, Ny, 0.7, 0.7, domain=40, seed=5)
+    for name, a in zip("abc", f):
+        write_density(tmp_path / f"{name}.n", a, 0.7, 0.7)
+    (tmp_path / "h.in").write_text(HETERO)
+    rc, out, err = run_cli(tmp_path, "h.in", env={"PFC_PRECISION": "mixed"} if mixed else None)
+    assert rc == 0, err
+    log = parse_log(out, 3)
+    o = Oracle(hetero_params(), Nx, Ny, 0.7, 0.7)
+    o.set_fields(f)
+    olog, snaps = o.relax(100, 50, snap_steps=(0, 50, 100))
+    tol = 2e-6 if mixed else 1e-11
+    np.testing.assert_allclose(log[:, 3], olog[:, 3], rtol=tol)
+    np.testing.assert_allclose(log[:, 4:], olog[:, 4:], rtol=tol, atol=1e-7 if mixed else 1e-14)
+    for k, t in enumerate((0, 50, 100)):
+        got = np.stack([read_density(tmp_path / f"{n}-{t}.n")[1] for n in "abc"])
+        err_l2 = np.linalg.norm(got - snaps[k]) / np.linalg.norm(snaps[k])
+        assert err_l2 < (1e-5 if mixed else 1e-10), (t, err_l2)
+    hdr, e = read_density(tmp_path / "e-100.f")
+    assert np.isfinite(e).all()
+
+
+def test_nan_input_exits_39(gpu, tmp_path):
+    Nx = Ny = 64
+    dx = LATTICE_A / 8
+    f = crystal(Nx, Ny, dx, dx, 0.2, -0.2, LATTICE_A, noise=0.01, seed=1)
+    write_density(tmp_path / "in.n", f, dx, dx)
+    txt = (tmp_path / "in.n").read_text().splitlines()
+    txt[1] = "nan"
+    (tmp_path / "in.n").write_text("\n".join(txt) + "\n")
+    (tmp_path / "run.in").write_text(HONEY.format(c=0, dx=repr(dx), dy=repr(dx), topt=0))
+    rc, out, err = run_cli(tmp_path, "run.in")
+    # t = 0 prints (with nan values) and writes, then step 0 fails
+    assert rc == 39, (out, err)
+    assert out.splitlines()[-1] == "Error: Not a number value detected."
+    assert (tmp_path / "out-0.n").exists() and not (tmp_path / "out-100.n").exists()
