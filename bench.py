@@ -10,9 +10,11 @@ resident in HBM and is bracketed by stream synchronisation.  A "step" is one
 pass of the semi-implicit update over the grid (the reference's step(),
 src/pfc-relax.c:1133-1247, including its every-100-steps refresh).
 
-Prints one JSON line.  `roofline` is for the dominant kernel (the column
-pass), timed with HIP events on the engine's stream over the timed region;
-`cpu_baseline` times the CPU oracle on a bounded sample of the same workload.
+Prints one JSON line.  `roofline` is for the dominant kernel (the longer of
+the row and column passes of the mixed step), timed with HIP events on the
+engine's stream over the timed region; `mixed_vs_fp64_run` compares the two
+forms from the same start over the workload's 5000 steps; `cpu_baseline`
+times the CPU oracle on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -70,6 +72,35 @@ def run_gpu(params, fields, Nx, Ny, dx, dy, precision, steps, warmup):
     return g, wall, wall_plain, ms, n
 
 
+def accuracy_study(params, fields, Nx, Ny, dx, dy, nsteps, checkpoints):
+    """Mixed form against the FP64 form from the same start over the
+    workload's full length: relative L2 of the densities and relative error
+    of the free energy per area at each checkpoint."""
+    from pfc_amd import FP64, MIXED
+    from pfc_amd.engine import Engine
+    gm = Engine(params, Nx, Ny, dx, dy, MIXED)
+    gd = Engine(params, Nx, Ny, dx, dy, FP64)
+    gm.set_fields(fields)
+    gd.set_fields(fields)
+    t = 0
+    rows = []
+    for c in checkpoints:
+        if c > nsteps:
+            break
+        gm.step(t, c - t)
+        gd.step(t, c - t)
+        t = c
+        fm, _ = gm.evaluate()
+        fd, _ = gd.evaluate()
+        xm, xd = gm.get_fields(), gd.get_fields()
+        rows.append({"step": c, "energy_per_area_rel": abs(fm - fd) / abs(fd),
+                     "density_rel_l2": float(np.linalg.norm(xm - xd) / np.linalg.norm(xd)),
+                     "fave_fp64": fd})
+    gm.close()
+    gd.close()
+    return rows
+
+
 def cpu_baseline(params, fields, Nx, Ny, dx, dy, budget_s):
     """Times the FP64 CPU oracle (OpenMP) on a bounded number of steps."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -102,6 +133,8 @@ def main(argv=None):
     ap.add_argument("--size", type=int, default=4096)
     ap.add_argument("--cpu-budget", type=float, default=15.0,
                     help="seconds of CPU-oracle stepping for the baseline (0: skip)")
+    ap.add_argument("--accuracy-steps", type=int, default=5000,
+                    help="length of the mixed-vs-FP64 comparison run (0: skip)")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported "
@@ -143,16 +176,24 @@ def main(argv=None):
     err_l2 = float(np.linalg.norm(m["fields"] - d["fields"]) / np.linalg.norm(d["fields"]))
     err_f = abs(m["fave"] - d["fave"]) / abs(d["fave"])
 
+    acc = None
+    if a.accuracy_steps > 0:
+        acc = accuracy_study(params, fields, Nx, Ny, dx, dy, a.accuracy_steps,
+                             (100, 500, 1000, 2000, 5000, 10000, 20000))
+
     cpu = None
     if a.cpu_budget > 0:
         cpu = cpu_baseline(params, fields, Nx, Ny, dx, dy, a.cpu_budget)
 
+    # the roofline is reported for the pass that takes longer in the mixed step
+    dom = "column" if m["col_ms"] >= m["row_ms"] else "row"
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "traffic_col_step_4096x4096_u3.txt")
+    tf = os.path.join(ROOT, "profiles", f"traffic_step_{Nx}x{Ny}_u3.txt")
     if os.path.exists(tf):
         for line in open(tf):
-            if line.startswith("mixed_bytes_per_launch:"):
+            if line.startswith(f"mixed_{dom}_bytes_per_launch:"):
                 traffic = float(line.split(":")[1])
+    achieved = m["col_gbs"] if dom == "column" else m["row_gbs"]
 
     out = {
         "metric": "grid-point-steps/s at 4096² 3-field, mixed vs FP64 on 1 GPU; HBM GB/s",
@@ -179,9 +220,11 @@ def main(argv=None):
         "speedup_mixed_vs_fp64": m["gps"] / d["gps"],
         "mixed_vs_fp64_error": {"density_rel_l2": err_l2, "energy_per_area_rel": err_f,
                                 "after_steps": a.warmup + 2 * a.steps},
-        "roofline": {"bound": "hbm", "achieved": m["col_gbs"], "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": m["col_gbs"] / PEAK_HBM_GBS, "traffic": traffic,
-                     "kernel": "k_col_step (column pass, mixed)"},
+        "mixed_vs_fp64_run": acc,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
+                     "kernel": ("k_col_step (column pass, mixed)" if dom == "column"
+                                else "k_row_step (row pass, mixed)")},
         "cpu_baseline": cpu,
         "input_generation_s": gen_s,
     }

@@ -161,9 +161,12 @@ __device__ __forceinline__ void gstore(GBuf b, unsigned voff, unsigned soff, cpx
 template <>
 __device__ __forceinline__ void gstore<double>(GBuf b, unsigned voff, unsigned soff,
                                                cpx<double> x) {
-#if PFC_STORE_B128
+#if PFC_STORE_B128 == 1
     using V = decltype(__builtin_amdgcn_raw_buffer_load_b128(b.r, 0, 0, 0));
     __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(V, x), b.r, (int)voff, (int)soff, 0);
+#elif PFC_STORE_B128 == 2
+    using V = decltype(__builtin_amdgcn_raw_buffer_load_b128(b.r, 0, 0, 0));
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(V, x), b.r, (int)(voff + soff), 0, 0);
 #else
     // Two 64-bit stores of the same bytes instead of one 128-bit store: a
     // 128-bit buffer store with an SGPR soffset whose data VGPRs are rewritten
@@ -211,8 +214,8 @@ struct BlockFFT {
     static constexpr int S = ilog(P, L);          // number of radix-P stages
     static constexpr int R0 = L / ipow(P, S);     // leading stage radix (1 if none)
 
-    __device__ static __forceinline__ cpx<T> twid(GBuf tw, int m) {
-        cpx<T> w = gload<T>(tw, (unsigned)(m * TWS) * sizeof(cpx<T>), 0u);
+    __device__ static __forceinline__ cpx<T> twid(GBuf tw, int m, unsigned z) {
+        cpx<T> w = gload<T>(tw, (unsigned)(m * TWS) * sizeof(cpx<T>), z);
         if constexpr (DIR > 0) w.y = -w.y;
         return w;
     }
@@ -220,7 +223,7 @@ struct BlockFFT {
     // One radix-R Stockham stage on butterflies j = t + b*TPB, inputs in in[].
     template <int R, bool LAST>
     __device__ static __forceinline__ void stage(cpx<T> (&in)[P], int Ns, cpx<T>* lds, GBuf tw,
-                                                 int t) {
+                                                 int t, unsigned z) {
         constexpr int B = P / R;
 #pragma unroll
         for (int b = 0; b < B; b++) {
@@ -235,7 +238,7 @@ struct BlockFFT {
                     // FP64: powers w^r of one table twiddle by recurrence
                     // (error grows by ~r ulp, r < 16), far fewer loads and
                     // live registers than one load per r
-                    const cpx<T> w1 = twid(tw, k * step);
+                    const cpx<T> w1 = twid(tw, k * step, z);
                     cpx<T> w = w1;
 #pragma unroll
                     for (int r = 1; r < R; r++) {
@@ -246,8 +249,8 @@ struct BlockFFT {
                     // FP32: two table twiddles w and w^(R/2); every other
                     // power is at most two products away (~2 ulp)
                     constexpr int Hh = R / 2;
-                    const cpx<T> w1 = twid(tw, k * step);
-                    const cpx<T> wh = twid(tw, k * Hh * step);
+                    const cpx<T> w1 = twid(tw, k * step, z);
+                    const cpx<T> wh = twid(tw, k * Hh * step, z);
                     cpx<T> lo[Hh];
                     lo[0] = {(T)1, (T)0};
                     lo[1] = w1;
@@ -260,7 +263,7 @@ struct BlockFFT {
                     }
                 } else {
 #pragma unroll
-                    for (int r = 1; r < R; r++) x[r] = cmul(x[r], twid(tw, k * r * step));
+                    for (int r = 1; r < R; r++) x[r] = cmul(x[r], twid(tw, k * r * step, z));
                 }
             }
             dft<R, DIR, T>(x);
@@ -289,12 +292,17 @@ struct BlockFFT {
 
     // v: natural-order input (thread t holds t + q*TPB); same layout on exit.
     // Begins with a barrier so lds may still be in use by the caller.
+    //
+    // The twiddle loads take an opaque zero offset, so that the compiler does
+    // not share them (and keep them live) across the transforms of a kernel.
     __device__ static __forceinline__ void run(cpx<T> (&v)[P], cpx<T>* lds, GBuf tw, int t) {
         int Ns = 1;
+        unsigned z = 0;
+        asm volatile("" : "+s"(z));
         __syncthreads();
         if constexpr (R0 > 1) {
             // in-register leading stage: butterfly b of thread t uses v[b + r*(P/R0)]
-            stage<R0, false>(v, 1, lds, tw, t);
+            stage<R0, false>(v, 1, lds, tw, t, z);
             Ns = R0;
             __syncthreads();
             gather<P>(v, lds, t);
@@ -302,10 +310,10 @@ struct BlockFFT {
 #pragma unroll
         for (int s = 0; s < S; s++) {
             if (s == S - 1) {
-                stage<P, true>(v, Ns, lds, tw, t);
+                stage<P, true>(v, Ns, lds, tw, t, z);
             } else {
                 if (s > 0 || R0 > 1) __syncthreads();
-                stage<P, false>(v, Ns, lds, tw, t);
+                stage<P, false>(v, Ns, lds, tw, t, z);
                 Ns *= P;
                 __syncthreads();
                 gather<P>(v, lds, t);

@@ -8,6 +8,7 @@
 //   mreal         U*nr real    densities in real space (materialised on demand)
 //   R             (2U + U(U-1)/2)*nr real  evaluate's real-space outputs
 //   f             nr doubles   energy density of the last evaluate
+//   ktab          (Nx/2+1 + Ny) E  {k^2, exp(-k^2/sigma^2)} per kx, then per ky
 // Everything runs on one HIP stream owned by the context.
 #include <hip/hip_runtime.h>
 
@@ -84,17 +85,44 @@ void fill_devparams(pfc_ctx* c) {
     }
 }
 
+// k tables of the column pass: for kx = 0..Nx/2 and then ky (j < Ny/2: j dky,
+// else (j - Ny) dky) the pair {k^2, exp(2 a k^2)} with a = -0.5/sigma^2, so
+// that k2 = kx^2 + ky^2 rounds as in the reference (src/pfc-relax.c:1217-1225)
+// and G2 = exp(a k2)^2 is the product of the two second entries.
+template <typename T>
+void fill_ktab(pfc_ctx* c) {
+    const int H = c->Nx / 2;
+    cpx<T>* k = (cpx<T>*)c->hktab;
+    const double a2 = 2.0 * c->hp.sig_a;
+    for (int i = 0; i <= H; i++) {
+        double kx = i * c->hp.dkx;
+        kx *= kx;
+        k[i] = {(T)kx, (T)std::exp(a2 * kx)};
+    }
+    for (int j = 0; j < c->Ny; j++) {
+        const double ky = (j < c->Ny / 2) ? j * c->hp.dky : (j - c->Ny) * c->hp.dky;
+        const double ky2 = ky * ky;
+        k[H + 1 + j] = {(T)ky2, (T)std::exp(a2 * ky2)};
+    }
+}
+
 int upload_params(pfc_ctx* c) {
+    HIPCHK(hipStreamSynchronize(c->stream));   // the host copies below are reused
     fill_devparams(c);
+    if (c->prec == PFC_FP64) fill_ktab<double>(c);
+    else fill_ktab<float>(c);
     HIPCHK(hipMemcpyAsync(c->dprm, &c->hp, sizeof(DevParams), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->ktab, c->hktab, (size_t)(c->Nx / 2 + 1 + c->Ny) * c->esz,
+                          hipMemcpyHostToDevice, c->stream));
     return PFC_OK;
 }
 
 void free_all(pfc_ctx* c) {
     void* bufs[] = {c->I, c->J, c->K, c->Nh, c->O, c->mreal, c->R, c->twx, c->twy,
-                    c->f, c->partials, c->sums, c->nan_flag, c->dprm};
+                    c->ktab, c->f, c->partials, c->sums, c->nan_flag, c->dprm};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
+    free(c->hktab);
     for (int i = 0; i < c->nev; i++) (void)hipEventDestroy(c->ev[i]);
     free(c->ev);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -125,11 +153,11 @@ int dalloc(void** p, size_t bytes) {
 // Materialise the densities in mreal; if K, also refresh Nh from them.
 int materialise(pfc_ctx* c, bool refresh) {
     if (c->prec == PFC_FP64) {
-        Launch<double>::row_c2r_any(c, c->I, c->mreal, refresh ? c->K : nullptr, c->U);
-        if (refresh) Launch<double>::col_fwd_any(c, c->K, c->Nh, c->U, c->hp.inv_NxNy);
+        Launch<double>::row_c2r_any(c, c->I, c->sI, c->mreal, refresh ? c->K : nullptr, c->U);
+        if (refresh) Launch<double>::col_fwd_any(c, c->K, c->sJ, c->Nh, c->U, c->hp.inv_NxNy);
     } else {
-        Launch<float>::row_c2r_any(c, c->I, c->mreal, refresh ? c->K : nullptr, c->U);
-        if (refresh) Launch<float>::col_fwd_any(c, c->K, c->Nh, c->U, c->hp.inv_NxNy);
+        Launch<float>::row_c2r_any(c, c->I, c->sI, c->mreal, refresh ? c->K : nullptr, c->U);
+        if (refresh) Launch<float>::col_fwd_any(c, c->K, c->sJ, c->Nh, c->U, c->hp.inv_NxNy);
     }
     HIPCHK(hipGetLastError());
     return PFC_OK;
@@ -176,6 +204,11 @@ int pfc_create(pfc_ctx** out, const pfc_params* p, int Nx, int Ny, int precision
     c->nkn = (size_t)(Nx / 2 + 1) * Ny;   // half spectrum, kx = 0..Nx/2
     c->nr = (size_t)Nx * Ny;
     c->esz = (precision == PFC_FP64) ? 16 : 8;
+    c->sC = ilog2(Ny);
+    c->sI = ilog2(precision == PFC_FP64 ? row_block<double>(Nx, Ny, PFC_I_BLOCKED)
+                                        : row_block<float>(Nx, Ny, PFC_I_BLOCKED));
+    c->sJ = ilog2(precision == PFC_FP64 ? row_block<double>(Nx, Ny, PFC_J_BLOCKED)
+                                        : row_block<float>(Nx, Ny, PFC_J_BLOCKED));
     const size_t rsz = c->esz / 2;
     c->dx = 1.0;
     c->dy = 1.0;
@@ -200,6 +233,9 @@ int pfc_create(pfc_ctx** out, const pfc_params* p, int Nx, int Ny, int precision
     rc |= dalloc((void**)&c->dprm, sizeof(DevParams));
     rc |= dalloc(&c->twx, Nx * c->esz);
     rc |= dalloc(&c->twy, Ny * c->esz);
+    rc |= dalloc(&c->ktab, (size_t)(Nx / 2 + 1 + Ny) * c->esz);
+    c->hktab = malloc((size_t)(Nx / 2 + 1 + Ny) * c->esz);
+    if (!c->hktab) rc |= PFC_ENOMEM;
     if (rc != PFC_OK) {
         free_all(c);
         free(c);
@@ -254,16 +290,16 @@ int pfc_set_fields(pfc_ctx* c, const double* fields) {
         HIPCHK(hipMemcpyAsync(c->mreal, fields, n * sizeof(double), hipMemcpyHostToDevice,
                               c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
-        Launch<double>::row_r2c_any(c, c->mreal, c->I, c->U);
-        Launch<double>::col_fwd_any(c, c->I, c->Nh, c->U, 1.0 / c->Ny);  // I carries 1/Nx
+        Launch<double>::row_r2c_any(c, c->mreal, c->I, c->sI, c->U);
+        Launch<double>::col_fwd_any(c, c->I, c->sI, c->Nh, c->U, 1.0 / c->Ny);  // I carries 1/Nx
     } else {
         std::vector<float> tmp(n);
         for (size_t i = 0; i < n; i++) tmp[i] = (float)fields[i];
         HIPCHK(hipMemcpyAsync(c->mreal, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice,
                               c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
-        Launch<float>::row_r2c_any(c, c->mreal, c->I, c->U);
-        Launch<float>::col_fwd_any(c, c->I, c->Nh, c->U, 1.0 / c->Ny);  // I carries 1/Nx
+        Launch<float>::row_r2c_any(c, c->mreal, c->I, c->sI, c->U);
+        Launch<float>::col_fwd_any(c, c->I, c->sI, c->Nh, c->U, 1.0 / c->Ny);  // I carries 1/Nx
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -321,14 +357,14 @@ int pfc_evaluate(pfc_ctx* c, double* fave, double* nave) {
     if (!c || !fave || !nave) return PFC_EINVAL;
     if (!c->have_fields || !c->have_box) return PFC_ESTATE;
     if (c->prec == PFC_FP64) {
-        Launch<double>::row_c2r_any(c, c->I, c->mreal, c->K, c->U);
+        Launch<double>::row_c2r_any(c, c->I, c->sI, c->mreal, c->K, c->U);
         Launch<double>::col_eval_any(c);
-        Launch<double>::row_c2r_any(c, c->O, c->R, nullptr, c->nO);
+        Launch<double>::row_c2r_any(c, c->O, c->sC, c->R, nullptr, c->nO);
         Launch<double>::energy_any(c);
     } else {
-        Launch<float>::row_c2r_any(c, c->I, c->mreal, c->K, c->U);
+        Launch<float>::row_c2r_any(c, c->I, c->sI, c->mreal, c->K, c->U);
         Launch<float>::col_eval_any(c);
-        Launch<float>::row_c2r_any(c, c->O, c->R, nullptr, c->nO);
+        Launch<float>::row_c2r_any(c, c->O, c->sC, c->R, nullptr, c->nO);
         Launch<float>::energy_any(c);
     }
     hipLaunchKernelGGL((k_reduce_final<1024>), dim3(1), dim3(1024), 0, c->stream, c->partials,

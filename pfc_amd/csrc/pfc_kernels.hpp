@@ -24,14 +24,25 @@ namespace pfc {
 
 constexpr int kMaxU = 4;   // fields supported by the fused kernels
 
-// Occupancy targets (waves per SIMD) handed to the register allocator.
+// Occupancy targets (waves per SIMD) handed to the register allocator, per
+// element precision.
 #define PFC_WAVES(n) __attribute__((amdgpu_waves_per_eu(n)))
-#ifndef PFC_ROW_WAVES
-#define PFC_ROW_WAVES 2
+#ifndef PFC_ROW_WAVES_F64
+#define PFC_ROW_WAVES_F64 2
 #endif
-#ifndef PFC_COL_WAVES
-#define PFC_COL_WAVES 2
+#ifndef PFC_ROW_WAVES_F32
+#define PFC_ROW_WAVES_F32 2
 #endif
+#ifndef PFC_COL_WAVES_F64
+#define PFC_COL_WAVES_F64 2
+#endif
+#ifndef PFC_COL_WAVES_F32
+#define PFC_COL_WAVES_F32 2
+#endif
+template <typename T>
+constexpr int row_waves() { return sizeof(T) == 8 ? PFC_ROW_WAVES_F64 : PFC_ROW_WAVES_F32; }
+template <typename T>
+constexpr int col_waves() { return sizeof(T) == 8 ? PFC_COL_WAVES_F64 : PFC_COL_WAVES_F32; }
 
 // Parameters in device memory, read with uniform (scalar) loads.
 struct DevParams {
@@ -62,9 +73,94 @@ __device__ __forceinline__ double ky_of(int j, int Ny, double dky) {
     return (j < Ny / 2) ? j * dky : (j - Ny) * dky;
 }
 
+// Row-spectral layouts.  A buffer of H columns by Ny rows is stored in
+// y-blocks of YB = 2^s rows: element (kx, y) at ((y >> s) H + kx) YB + (y mod YB).
+// s = log2(Ny) is the column-major layout kx Ny + y (whole columns
+// contiguous); s = log2(TR) makes the TR rows of a row-pass block one
+// contiguous run of TR elements per kx.
+// Row-pass lane (kx = t + q TPB, row y): byte offset of q = 0 and the step in q.
+__device__ __forceinline__ unsigned lay_row_off(int s, int H, int t, int y, unsigned E) {
+    return (((unsigned)(y >> s) * (unsigned)H + (unsigned)t) << s | (unsigned)(y & ((1 << s) - 1))) * E;
+}
+__device__ __forceinline__ unsigned lay_row_step(int s, int TPB, unsigned E) {
+    return ((unsigned)TPB << s) * E;
+}
+// Column-pass lane (column kx, y = t + q TPB); needs TPB >= YB or YB >= Ny.
+__device__ __forceinline__ unsigned lay_col_off(int s, int H, int kx, int t, unsigned E) {
+    return (((unsigned)(t >> s) * (unsigned)H + (unsigned)kx) << s | (unsigned)(t & ((1 << s) - 1))) * E;
+}
+__device__ __forceinline__ unsigned lay_col_step(int s, int H, int TPB, unsigned E) {
+    return (TPB >= (1 << s)) ? (unsigned)TPB * (unsigned)H * E : (unsigned)TPB * E;
+}
+
+// ------------------------------------------------ elementary functions
+//
+// exp and expm1 for the operators A and B, written for this kernel: Cody-Waite
+// argument reduction x = n ln2 + r with |r| <= ln2/2 (ln2 split so that n*hi
+// is exact), the Taylor series of e^r (resp. e^r - 1), and a scale by 2^n.
+// No special-case branches: the argument is clamped to where the result
+// underflows to 0 or overflows to inf (NaN passes through the clamps).
+
+// FP64 exp: degree-13 Taylor series (truncation < 5e-18 relative on |r| <=
+// 0.3466), within about 1 ulp of the correctly rounded value.
+__device__ __forceinline__ double exp_cw(double x) {
+    x = x < -746.0 ? -746.0 : x;
+    x = x > 710.0 ? 710.0 : x;
+    const double n = rint(x * 1.4426950408889634074);
+    double r = fma(-n, 6.93147180369123816490e-01, x);   // ln2 high part (21 trailing zero bits)
+    r = fma(-n, 1.90821492927058770002e-10, r);           // ln2 low part
+    double q = 1.0 / 6227020800.0;                        // 1/13!
+    q = fma(q, r, 1.0 / 479001600.0);
+    q = fma(q, r, 1.0 / 39916800.0);
+    q = fma(q, r, 1.0 / 3628800.0);
+    q = fma(q, r, 1.0 / 362880.0);
+    q = fma(q, r, 1.0 / 40320.0);
+    q = fma(q, r, 1.0 / 5040.0);
+    q = fma(q, r, 1.0 / 720.0);
+    q = fma(q, r, 1.0 / 120.0);
+    q = fma(q, r, 1.0 / 24.0);
+    q = fma(q, r, 1.0 / 6.0);
+    q = fma(q, r, 0.5);
+    q = fma(q, r, 1.0);
+    q = fma(q, r, 1.0);
+    return ldexp(q, (int)n);
+}
+
+// FP32 expm1: degree-8 series of e^r - 1 (truncation < 2e-8 relative), then
+// e^x - 1 = 2^n (e^r - 1) + (2^n - 1), which has no cancellation for n != 0.
+__device__ __forceinline__ float expm1_cw(float x) {
+    x = x < -104.0f ? -104.0f : x;
+    x = x > 89.0f ? 89.0f : x;
+    const float n = rintf(x * 1.44269504f);
+    float r = fmaf(-n, 0.693359375f, x);   // ln2 = 0.693359375 - 2.12194440e-4
+    r = fmaf(n, 2.12194440e-4f, r);
+    float q = 1.0f / 40320.0f;
+    q = fmaf(q, r, 1.0f / 5040.0f);
+    q = fmaf(q, r, 1.0f / 720.0f);
+    q = fmaf(q, r, 1.0f / 120.0f);
+    q = fmaf(q, r, 1.0f / 24.0f);
+    q = fmaf(q, r, 1.0f / 6.0f);
+    q = fmaf(q, r, 0.5f);
+    q = fmaf(q * r, r, r);                 // e^r - 1
+    const float s = ldexpf(1.0f, (int)n);
+    return fmaf(s, q, s - 1.0f);
+}
+
+// 1/a: hardware reciprocal estimate refined by two Newton steps (FP64) or
+// the hardware reciprocal as is (FP32, 1 ulp).
+__device__ __forceinline__ double rcp_nr(double a) {
+    double r = __builtin_amdgcn_rcp(a);
+    double e = fma(-a, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-a, r, 1.0);
+    return fma(r, e, r);
+}
+__device__ __forceinline__ float rcp_nr(float a) { return __builtin_amdgcn_rcpf(a); }
+
 // ---------------------------------------------------------------- k space
 
-// A and B of update_AB (src/pfc-relax.c:940-953) for field u at k2.
+// A and B of update_AB (src/pfc-relax.c:940-953) for field u at k2:
+// A = exp(o a dt), B = (A - 1)/a/(Nx Ny), or o dt/(Nx Ny) where a == 0.
 __device__ __forceinline__ void op_AB(const DevParams* __restrict__ p, int u, double k2,
                                       double& A, double& B) {
     double P = 1.0;
@@ -74,14 +170,15 @@ __device__ __forceinline__ void op_AB(const DevParams* __restrict__ p, int u, do
     }
     const double a = p->alpha[u][u] + p->beta[u][u] * P;
     const double o = p->c1 - 1.0 - p->c1 * k2;
-    const double ex = exp(o * a * p->dt);
+    const double ex = exp_cw(o * a * p->dt);
     A = ex;
-    const double b = (a == 0.0) ? o * p->dt : (ex - 1.0) / a;
+    const double b = (a == 0.0) ? o * p->dt : (ex - 1.0) * rcp_nr(a);
     B = b * p->inv_NxNy;
 }
 
-// FP32 variant: same operator, B from expm1 so that the long-wavelength
-// modes, where exp(x)-1 cancels to a few bits in FP32, keep full precision.
+// FP32 variant: same operator, with e^x - 1 from expm1 so that the
+// long-wavelength modes, where exp(x) - 1 cancels to a few bits in FP32, keep
+// full precision.
 __device__ __forceinline__ void op_AB(const DevParams* __restrict__ p, int u, float k2,
                                       float& A, float& B) {
     float P = 1.0f;
@@ -92,23 +189,21 @@ __device__ __forceinline__ void op_AB(const DevParams* __restrict__ p, int u, fl
     const float a = (float)p->alpha[u][u] + (float)p->beta[u][u] * P;
     const float c1 = (float)p->c1;
     const float o = c1 - 1.0f - c1 * k2;
-    const float x = o * a * (float)p->dt;
-    const float em1 = expm1f(x);
+    const float em1 = expm1_cw(o * a * (float)p->dt);
     A = 1.0f + em1;
-    const float b = (a == 0.0f) ? o * (float)p->dt : em1 / a;
+    const float b = (a == 0.0f) ? o * (float)p->dt : em1 * rcp_nr(a);
     B = b * (float)p->inv_NxNy;
 }
 
 // The k-space update of step() (src/pfc-relax.c:1214-1242) at one k point:
 // explicit couplings into M, then N <- A N + B M.  On return N holds the new
-// state (the reference also copies it into M).
+// state (the reference also copies it into M).  G2 = exp(-k2/(2 sigma^2))^2,
+// passed in (it is separable in kx and ky and comes from tables).
 template <int U, typename T>
-__device__ __forceinline__ void kupdate(const DevParams* __restrict__ p, T k2, cpx<T> (&M)[U],
-                                        cpx<T> (&N)[U]) {
+__device__ __forceinline__ void kupdate(const DevParams* __restrict__ p, T k2, T G2,
+                                        cpx<T> (&M)[U], cpx<T> (&N)[U]) {
     const T NxNy = (T)p->NxNy;
     if constexpr (U > 1) {
-        T G2 = exp((T)p->sig_a * k2);
-        G2 *= G2;
 #pragma unroll
         for (int u = 0; u < U - 1; u++) {
 #pragma unroll
@@ -123,6 +218,8 @@ __device__ __forceinline__ void kupdate(const DevParams* __restrict__ p, T k2, c
                 M[v].y += (bd * N[u].y + eg * N[u].y) * NxNy;
             }
         }
+    } else {
+        (void)G2;
     }
 #pragma unroll
     for (int u = 0; u < U; u++) {
@@ -225,10 +322,10 @@ __device__ __forceinline__ void nonlinear(const DevParams* __restrict__ p, T (&m
 // and adjacent addresses in the column-major buffers (TR*sizeof(E) bytes
 // contiguous per kx).
 template <typename T, int NX, int TPB, int TR, int U, bool REFRESH>
-__global__ __launch_bounds__(TPB* TR) PFC_WAVES(PFC_ROW_WAVES) void k_row_step(
+__global__ __launch_bounds__(TPB* TR) PFC_WAVES(row_waves<T>()) void k_row_step(
     const cpx<T>* __restrict__ I, cpx<T>* __restrict__ J, cpx<T>* __restrict__ K,
-    size_t fstride, int Ny, const cpx<T>* __restrict__ twx_, const DevParams* __restrict__ prm,
-    int* __restrict__ nan_step, int tstep) {
+    size_t fstride, int sI, int sJ, const cpx<T>* __restrict__ twx_,
+    const DevParams* __restrict__ prm, int* __restrict__ nan_step, int tstep) {
     constexpr int H = NX / 2;
     constexpr int P = H / TPB;
     constexpr unsigned E = sizeof(cpx<T>);
@@ -241,15 +338,15 @@ __global__ __launch_bounds__(TPB* TR) PFC_WAVES(PFC_ROW_WAVES) void k_row_step(
     const int y = xcd_remap(blockIdx.x, gridDim.x) * TR + r;
     cpx<T>* L = lds + r;
     const GBuf twx = gbuf(twx_);
-    const unsigned voff = ((unsigned)t * (unsigned)Ny + (unsigned)y) * E;
-    const unsigned qstep = (unsigned)TPB * (unsigned)Ny * E;
+    const unsigned ioff = lay_row_off(sI, H, t, y, E), istep = lay_row_step(sI, TPB, E);
+    const unsigned voff = lay_row_off(sJ, H, t, y, E), qstep = lay_row_step(sJ, TPB, E);
 
     cpx<T> m[U][P];
 #pragma unroll
     for (int u = 0; u < U; u++) {
         const GBuf src = gbuf(I + u * fstride);
 #pragma unroll
-        for (int q = 0; q < P; q++) m[u][q] = gload<T>(src, voff, q * qstep);
+        for (int q = 0; q < P; q++) m[u][q] = gload<T>(src, ioff, q * istep);
     }
 #pragma unroll
     for (int u = 0; u < U; u++) {
@@ -356,13 +453,21 @@ __device__ __forceinline__ void hermitian_cols(cpx<T> (&M)[U][NY / TPB], cpx<T>*
 // Column pass of a step: TC columns per block (lane = column-major within the
 // block), each column all Ny rows; the extra last block runs kx = H.
 // fstride: row-spectral field stride (H*Ny), nstride: state stride ((H+1)*Ny).
+// ktab: H+1 entries {kx^2, exp(-kx^2/sigma^2)} then Ny entries for ky, so
+// k2 = kx^2 + ky^2 rounds as in the reference and G2 is their product.
+// PREF issues the state loads before the forward transforms.
+#ifndef PFC_COL_PREF
+#define PFC_COL_PREF 0
+#endif
 template <typename T, int NY, int TPB, int TC, int U, bool REFRESH>
-__global__ __launch_bounds__(TPB* TC) PFC_WAVES(PFC_COL_WAVES) void k_col_step(
+__global__ __launch_bounds__(TPB* TC) PFC_WAVES(col_waves<T>()) void k_col_step(
     const cpx<T>* __restrict__ J, const cpx<T>* __restrict__ K, cpx<T>* __restrict__ Nh,
-    cpx<T>* __restrict__ I, cpx<T>* __restrict__ Nr, size_t fstride, size_t nstride, int H,
-    const cpx<T>* __restrict__ twy_, const DevParams* __restrict__ prm) {
+    cpx<T>* __restrict__ I, size_t fstride, size_t nstride, int H, int sI, int sJ,
+    const cpx<T>* __restrict__ twy_, const cpx<T>* __restrict__ ktab_,
+    const DevParams* __restrict__ prm) {
     constexpr int P = NY / TPB;
     constexpr unsigned E = sizeof(cpx<T>);
+    constexpr bool PREF = PFC_COL_PREF != 0 && !REFRESH;
     using FFTF = BlockFFT<T, NY, TPB, -1, 1, 1>;
     using FFTI = BlockFFT<T, NY, TPB, +1, 1, 1>;
     __shared__ cpx<T> lds[NY * TC];
@@ -370,57 +475,69 @@ __global__ __launch_bounds__(TPB* TC) PFC_WAVES(PFC_COL_WAVES) void k_col_step(
     const int t = threadIdx.x % TPB;
     cpx<T>* L = lds + c * NY;
     const GBuf twy = gbuf(twy_);
+    const GBuf ktab = gbuf(ktab_);
     {
         const ColJob jb = col_job(blockIdx.x, c, TC, H);
         const int kx = jb.kx;
         // byte offsets of this lane's first element in a field's buffers
-        const unsigned vr = ((unsigned)(jb.mode ? 0 : kx) * NY + (unsigned)t) * E;
+        const unsigned vr = lay_col_off(sJ, H, jb.mode ? 0 : kx, t, E);
+        const unsigned rstep = lay_col_step(sJ, H, TPB, E);
         const unsigned vn = ((unsigned)kx * NY + (unsigned)t) * E;
+        const unsigned vi = lay_col_off(sI, H, jb.mode ? 0 : kx, t, E);
+        const unsigned istep = lay_col_step(sI, H, TPB, E);
 
+        // state read by the update: Nh (loaded early when PREF)
+        cpx<T> S[U][PREF ? P : 1];
+        if constexpr (PREF) {
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const GBuf src = gbuf(Nh + u * nstride);
+#pragma unroll
+                for (int q = 0; q < P; q++) S[u][q] = gload<T>(src, vn, q * TPB * E);
+            }
+        }
         cpx<T> M[U][P];
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const GBuf src = gbuf(J + u * fstride);
 #pragma unroll
-            for (int q = 0; q < P; q++) M[u][q] = col_in<T>(src, vr, q * TPB * E, jb.mode);
+            for (int q = 0; q < P; q++) M[u][q] = col_in<T>(src, vr, q * rstep, jb.mode);
             FFTF::run(M[u], L, twy, t);
         }
-        // state read by the update: Nh, or on refresh steps N = FFT(n)/NxNy,
-        // computed here from the density's row transforms into the scratch Nr
-        // (so that no address of Nh is written twice by this kernel)
-        cpx<T>* const Nin = REFRESH ? Nr : Nh;
         if constexpr (REFRESH) {
-            const T s = (T)prm->inv_NxNy;
+            // N = FFT(n)/NxNy from the density's row transforms
+            // (src/pfc-relax.c:1178-1184), written to Nh and read back below
+            // by the same lanes
+            const T sc = (T)prm->inv_NxNy;
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 const GBuf src = gbuf(K + u * fstride);
-                const GBuf dst = gbuf(Nr + u * nstride);
+                const GBuf dst = gbuf(Nh + u * nstride);
                 cpx<T> w[P];
 #pragma unroll
-                for (int q = 0; q < P; q++) w[q] = col_in<T>(src, vr, q * TPB * E, jb.mode);
+                for (int q = 0; q < P; q++) w[q] = col_in<T>(src, vr, q * rstep, jb.mode);
                 FFTF::run(w, L, twy, t);
                 if (jb.live) {
 #pragma unroll
-                    for (int q = 0; q < P; q++) gstore<T>(dst, vn, q * TPB * E, cscale(w[q], s));
+                    for (int q = 0; q < P; q++) gstore<T>(dst, vn, q * TPB * E, cscale(w[q], sc));
                 }
             }
         }
         if (jb.live) {
-            const double dky = prm->dky;
-            const double kxd = kx * prm->dkx;
-            const double kx2d = kxd * kxd;
+            const cpx<T> kxe = gload<T>(ktab, (unsigned)kx * E, 0u);
 #pragma unroll
             for (int q = 0; q < P; q++) {
-                const int j = t + q * TPB;
-                const double ky = ky_of(j, NY, dky);
-                const T k2 = (T)(kx2d + ky * ky);
+                const cpx<T> kye = gload<T>(ktab, (unsigned)(H + 1 + t) * E, q * TPB * E);
+                const T k2 = kxe.x + kye.x;
+                const T G2 = kxe.y * kye.y;
                 cpx<T> Mk[U], Nk[U];
 #pragma unroll
                 for (int u = 0; u < U; u++) {
                     Mk[u] = M[u][q];
-                    Nk[u] = gload<T>(gbuf(Nin + u * nstride), vn, q * TPB * E);
+                    if constexpr (PREF) Nk[u] = S[u][q];
+                    else Nk[u] = gload<T>(gbuf(Nh + u * nstride), vn, q * TPB * E);
                 }
-                kupdate<U, T>(prm, k2, Mk, Nk);
+                kupdate<U, T>(prm, k2, G2, Mk, Nk);
 #pragma unroll
                 for (int u = 0; u < U; u++) {
                     gstore<T>(gbuf(Nh + u * nstride), vn, q * TPB * E, Nk[u]);
@@ -448,7 +565,7 @@ __global__ __launch_bounds__(TPB* TC) PFC_WAVES(PFC_COL_WAVES) void k_col_step(
             if (jb.live) {
                 const GBuf dst = gbuf(I + u * fstride);
 #pragma unroll
-                for (int q = 0; q < P; q++) col_out<T>(dst, vr, q * TPB * E, jb.mode, M[u][q]);
+                for (int q = 0; q < P; q++) col_out<T>(dst, vi, q * istep, jb.mode, M[u][q]);
             }
         }
     }
@@ -462,8 +579,8 @@ __global__ __launch_bounds__(TPB* TC) PFC_WAVES(PFC_COL_WAVES) void k_col_step(
 template <typename T, int NX, int TPB, int TR>
 __global__ __launch_bounds__(TPB* TR) void k_row_c2r(const cpx<T>* __restrict__ src,
                                                      T* __restrict__ dst, cpx<T>* __restrict__ K,
-                                                     size_t fstride, size_t rstride, int Ny,
-                                                     const cpx<T>* __restrict__ twx_) {
+                                                     size_t fstride, size_t rstride, int sS,
+                                                     int sK, const cpx<T>* __restrict__ twx_) {
     const GBuf twx = gbuf(twx_);
     constexpr int H = NX / 2;
     constexpr int P = H / TPB;
@@ -475,12 +592,12 @@ __global__ __launch_bounds__(TPB* TR) void k_row_c2r(const cpx<T>* __restrict__ 
     const int b = blockIdx.y;
     cpx<T>* L = lds + r;
     constexpr unsigned E = sizeof(cpx<T>);
-    const unsigned voff = ((unsigned)t * (unsigned)Ny + (unsigned)y) * E;
-    const unsigned qstep = (unsigned)TPB * (unsigned)Ny * E;
+    const unsigned soff = lay_row_off(sS, H, t, y, E), sstep = lay_row_step(sS, TPB, E);
+    const unsigned voff = lay_row_off(sK, H, t, y, E), qstep = lay_row_step(sK, TPB, E);
     cpx<T> v[P];
     const GBuf sb = gbuf(src + b * fstride);
 #pragma unroll
-    for (int q = 0; q < P; q++) v[q] = gload<T>(sb, voff, q * qstep);
+    for (int q = 0; q < P; q++) v[q] = gload<T>(sb, soff, q * sstep);
     c2r_pre<T, H, TPB, TR>(v, L, twx, t);
     BlockFFT<T, H, TPB, +1, 2, TR>::run(v, L, twx, t);
     const GBuf db = gbuf(dst + b * rstride);
@@ -501,7 +618,7 @@ __global__ __launch_bounds__(TPB* TR) void k_row_c2r(const cpx<T>* __restrict__ 
 template <typename T, int NX, int TPB, int TR>
 __global__ __launch_bounds__(TPB* TR) void k_row_r2c(const T* __restrict__ src,
                                                      cpx<T>* __restrict__ dst, size_t fstride,
-                                                     size_t rstride, int Ny,
+                                                     size_t rstride, int sD,
                                                      const cpx<T>* __restrict__ twx_, T scale) {
     const GBuf twx = gbuf(twx_);
     constexpr int H = NX / 2;
@@ -522,8 +639,7 @@ __global__ __launch_bounds__(TPB* TR) void k_row_r2c(const T* __restrict__ src,
     BlockFFT<T, H, TPB, -1, 2, TR>::run(v, L, twx, t);
     r2c_post<T, H, TPB, TR>(v, L, twx, t);
     const GBuf db = gbuf(dst + b * fstride);
-    const unsigned voff = ((unsigned)t * (unsigned)Ny + (unsigned)y) * E;
-    const unsigned qstep = (unsigned)TPB * (unsigned)Ny * E;
+    const unsigned voff = lay_row_off(sD, H, t, y, E), qstep = lay_row_step(sD, TPB, E);
 #pragma unroll
     for (int q = 0; q < P; q++) gstore<T>(db, voff, q * qstep, cscale(v[q], scale));
 }
@@ -534,7 +650,7 @@ __global__ __launch_bounds__(TPB* TR) void k_row_r2c(const T* __restrict__ src,
 template <typename T, int NY, int TPB, int TC>
 __global__ __launch_bounds__(TPB* TC) void k_col_fwd(const cpx<T>* __restrict__ src,
                                                      cpx<T>* __restrict__ dst, size_t fstride,
-                                                     size_t nstride, int H,
+                                                     size_t nstride, int H, int sS,
                                                      const cpx<T>* __restrict__ twy_,
                                                      double scale) {
     const GBuf twy = gbuf(twy_);
@@ -548,11 +664,12 @@ __global__ __launch_bounds__(TPB* TC) void k_col_fwd(const cpx<T>* __restrict__ 
     const GBuf sb = gbuf(src + b * fstride), db = gbuf(dst + b * nstride);
     {
         const ColJob jb = col_job(blockIdx.x, c, TC, H);
-        const unsigned vr = ((unsigned)(jb.mode ? 0 : jb.kx) * NY + (unsigned)t) * E;
+        const unsigned vr = lay_col_off(sS, H, jb.mode ? 0 : jb.kx, t, E);
+        const unsigned rstep = lay_col_step(sS, H, TPB, E);
         const unsigned vn = ((unsigned)jb.kx * NY + (unsigned)t) * E;
         cpx<T> v[P];
 #pragma unroll
-        for (int q = 0; q < P; q++) v[q] = col_in<T>(sb, vr, q * TPB * E, jb.mode);
+        for (int q = 0; q < P; q++) v[q] = col_in<T>(sb, vr, q * rstep, jb.mode);
         BlockFFT<T, NY, TPB, -1, 1, 1>::run(v, L, twy, t);
         if (jb.live) {
 #pragma unroll
@@ -590,7 +707,7 @@ template <typename T, int NY, int TPB, int TC>
 __global__ __launch_bounds__(TPB* TC) void k_col_eval(const cpx<T>* __restrict__ K,
                                                       cpx<T>* __restrict__ Nh,
                                                       cpx<T>* __restrict__ O, size_t fstride,
-                                                      size_t nstride, int H,
+                                                      size_t nstride, int H, int sK,
                                                       const cpx<T>* __restrict__ twy_,
                                                       const DevParams* __restrict__ prm) {
     const GBuf twy = gbuf(twy_);
@@ -606,11 +723,13 @@ __global__ __launch_bounds__(TPB* TC) void k_col_eval(const cpx<T>* __restrict__
     const GBuf kb = gbuf(K + s * fstride), nb = gbuf(Nh + s * nstride);
     {
         const ColJob jb = col_job(blockIdx.x, c, TC, H);
+        const unsigned vk = lay_col_off(sK, H, jb.mode ? 0 : jb.kx, t, E);
+        const unsigned kstep = lay_col_step(sK, H, TPB, E);
         const unsigned vr = ((unsigned)(jb.mode ? 0 : jb.kx) * NY + (unsigned)t) * E;
         const unsigned vn = ((unsigned)jb.kx * NY + (unsigned)t) * E;
         cpx<T> M[P];
 #pragma unroll
-        for (int q = 0; q < P; q++) M[q] = col_in<T>(kb, vr, q * TPB * E, jb.mode);
+        for (int q = 0; q < P; q++) M[q] = col_in<T>(kb, vk, q * kstep, jb.mode);
         BlockFFT<T, NY, TPB, -1, 1, 1>::run(M, L, twy, t);
 #pragma unroll
         for (int q = 0; q < P; q++) {

