@@ -51,7 +51,13 @@ void fill_twiddles(std::vector<cpx<T>>& tw, int n) {
 void fill_devparams(pfc_ctx* c) {
     DevParams& d = c->hp;
     const pfc_params& p = c->p;
+#if PFC_PROBE
+    unsigned long long* probe = d.probe;
+#endif
     std::memset(&d, 0, sizeof(d));
+#if PFC_PROBE
+    d.probe = probe;
+#endif
     d.U = p.U;
     for (int u = 0; u < p.U; u++) d.W[u] = p.W[u];
     d.energy_compat = p.energy_compat;
@@ -231,6 +237,9 @@ int pfc_create(pfc_ctx** out, const pfc_params* p, int Nx, int Ny, int precision
     rc |= dalloc((void**)&c->sums, (1 + c->U) * sizeof(double));
     rc |= dalloc((void**)&c->nan_flag, sizeof(int));
     rc |= dalloc((void**)&c->dprm, sizeof(DevParams));
+#if PFC_PROBE
+    rc |= dalloc((void**)&c->hp.probe, 2 * 65536 * sizeof(unsigned long long));
+#endif
     rc |= dalloc(&c->twx, Nx * c->esz);
     rc |= dalloc(&c->twy, Ny * c->esz);
     rc |= dalloc(&c->ktab, (size_t)(Nx / 2 + 1 + Ny) * c->esz);
@@ -336,6 +345,17 @@ int pfc_timing(pfc_ctx* c, int enable) {
     if (enable) c->nrec = 0;
     return PFC_OK;
 }
+
+#if PFC_PROBE
+// Diagnostic builds only: copies the phase timestamps of the last row and
+// column pass (2 x 65536 words).
+int pfc_probe_read(pfc_ctx* c, unsigned long long* host) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(host, c->hp.probe, 2 * 65536 * sizeof(unsigned long long),
+                     hipMemcpyDeviceToHost));
+    return PFC_OK;
+}
+#endif
 
 int pfc_timing_read(pfc_ctx* c, double* ms, int* nsteps) {
     if (!c || !ms || !nsteps) return PFC_EINVAL;

@@ -27,6 +27,9 @@ constexpr int kMaxU = 4;   // fields supported by the fused kernels
 // Occupancy targets (waves per SIMD) handed to the register allocator, per
 // element precision.
 #define PFC_WAVES(n) __attribute__((amdgpu_waves_per_eu(n)))
+#ifndef PFC_PROBE
+#define PFC_PROBE 0
+#endif
 #ifndef PFC_ROW_WAVES_F64
 #define PFC_ROW_WAVES_F64 2
 #endif
@@ -59,7 +62,30 @@ struct DevParams {
     double alpha[8][8], beta[8][8], gamma[8][8], epsilon[8][8], mu2[8][8];
     double qc2[8][8];  // coefficient of the energy's beta_uv gradient term
     double nu2[8][8], omega[8][8], phi[8][8];
+#if PFC_PROBE
+    unsigned long long* probe;   // per-block phase timestamps (diagnostic builds)
+#endif
 };
+
+#if PFC_PROBE
+// Diagnostic builds: lane 0 of each block records the 100 MHz wall clock at
+// phase boundaries (after a barrier) and its hardware ids into probe.
+#define PFC_PROBE_AT(base, k)                                                      \
+    do {                                                                           \
+        __syncthreads();                                                           \
+        if (threadIdx.x == 0) {                                                    \
+            prm->probe[(base) + (size_t)blockIdx.x * 8 + (k)] = wall_clock64();    \
+            if ((k) == 0)                                                          \
+                prm->probe[(base) + (size_t)blockIdx.x * 8 + 7] =                  \
+                    ((unsigned long long)__builtin_amdgcn_s_getreg(63508) << 32) | \
+                    (unsigned)__builtin_amdgcn_s_getreg(63492);                    \
+        }                                                                          \
+    } while (0)
+#else
+#define PFC_PROBE_AT(base, k) \
+    do {                      \
+    } while (0)
+#endif
 
 // Blocks are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8).
 // Remap so each XCD walks a contiguous range of logical indices: neighbouring
@@ -238,8 +264,12 @@ __device__ __forceinline__ void kupdate(const DevParams* __restrict__ p, T k2, T
 template <typename T, int H, int TPB, int LS>
 __device__ __forceinline__ void c2r_pre(cpx<T> (&v)[H / TPB], cpx<T>* lds, GBuf twx, int t) {
     constexpr int P = H / TPB;
-    // e^{-2 pi i k/Nx} for k = t + q*TPB is e^{-2 pi i t/Nx} e^{-2 pi i q/(2P)}
-    const cpx<T> wt = gload<T>(twx, (unsigned)t * sizeof(cpx<T>), 0u);
+    // e^{-2 pi i k/Nx} for k = t + q*TPB is e^{-2 pi i t/Nx} e^{-2 pi i q/(2P)};
+    // loaded through an opaque zero offset so that the compiler recomputes
+    // the products per call instead of keeping them live across the kernel
+    unsigned z = 0;
+    asm volatile("" : "+s"(z));
+    const cpx<T> wt = gload<T>(twx, (unsigned)t * sizeof(cpx<T>), z);
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < P; q++) lds[(t + q * TPB) * LS] = v[q];
@@ -266,7 +296,9 @@ __device__ __forceinline__ void c2r_pre(cpx<T> (&v)[H / TPB], cpx<T>* lds, GBuf 
 template <typename T, int H, int TPB, int LS>
 __device__ __forceinline__ void r2c_post(cpx<T> (&v)[H / TPB], cpx<T>* lds, GBuf twx, int t) {
     constexpr int P = H / TPB;
-    const cpx<T> wt = gload<T>(twx, (unsigned)t * sizeof(cpx<T>), 0u);
+    unsigned z = 0;
+    asm volatile("" : "+s"(z));
+    const cpx<T> wt = gload<T>(twx, (unsigned)t * sizeof(cpx<T>), z);
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < P; q++) lds[(t + q * TPB) * LS] = v[q];
@@ -341,6 +373,7 @@ __global__ __launch_bounds__(TPB* TR) PFC_WAVES(row_waves<T>()) void k_row_step(
     const unsigned ioff = lay_row_off(sI, H, t, y, E), istep = lay_row_step(sI, TPB, E);
     const unsigned voff = lay_row_off(sJ, H, t, y, E), qstep = lay_row_step(sJ, TPB, E);
 
+    PFC_PROBE_AT(0, 0);
     cpx<T> m[U][P];
 #pragma unroll
     for (int u = 0; u < U; u++) {
@@ -352,7 +385,9 @@ __global__ __launch_bounds__(TPB* TR) PFC_WAVES(row_waves<T>()) void k_row_step(
     for (int u = 0; u < U; u++) {
         c2r_pre<T, H, TPB, TR>(m[u], L, twx, t);
         FFTI::run(m[u], L, twx, t);
+        if (u == 0) PFC_PROBE_AT(0, 1);
     }
+    PFC_PROBE_AT(0, 2);
     // NaN check on field 0 at (0,0) (src/pfc-relax.c:1175)
     if (y == 0 && t == 0) {
         const T v0 = m[0][0].x;
@@ -385,6 +420,7 @@ __global__ __launch_bounds__(TPB* TR) PFC_WAVES(row_waves<T>()) void k_row_step(
 #pragma unroll
         for (int u = 0; u < U; u++) m[u][q] = {a[u], b[u]};
     }
+    PFC_PROBE_AT(0, 3);
 #pragma unroll
     for (int u = 0; u < U; u++) {
         FFTF::run(m[u], L, twx, t);
@@ -393,6 +429,7 @@ __global__ __launch_bounds__(TPB* TR) PFC_WAVES(row_waves<T>()) void k_row_step(
 #pragma unroll
         for (int q = 0; q < P; q++) gstore<T>(dst, voff, q * qstep, m[u][q]);
     }
+    PFC_PROBE_AT(0, 4);
 }
 
 // Column addressing modes of the column passes.  mode 0: an ordinary column
@@ -455,9 +492,11 @@ __device__ __forceinline__ void hermitian_cols(cpx<T> (&M)[U][NY / TPB], cpx<T>*
 // fstride: row-spectral field stride (H*Ny), nstride: state stride ((H+1)*Ny).
 // ktab: H+1 entries {kx^2, exp(-kx^2/sigma^2)} then Ny entries for ky, so
 // k2 = kx^2 + ky^2 rounds as in the reference and G2 is their product.
-// PREF issues the state loads before the forward transforms.
-#ifndef PFC_COL_PREF
-#define PFC_COL_PREF 0
+#ifndef PFC_COL_QB_F64
+#define PFC_COL_QB_F64 4
+#endif
+#ifndef PFC_COL_QB_F32
+#define PFC_COL_QB_F32 2
 #endif
 template <typename T, int NY, int TPB, int TC, int U, bool REFRESH>
 __global__ __launch_bounds__(TPB* TC) PFC_WAVES(col_waves<T>()) void k_col_step(
@@ -467,7 +506,6 @@ __global__ __launch_bounds__(TPB* TC) PFC_WAVES(col_waves<T>()) void k_col_step(
     const DevParams* __restrict__ prm) {
     constexpr int P = NY / TPB;
     constexpr unsigned E = sizeof(cpx<T>);
-    constexpr bool PREF = PFC_COL_PREF != 0 && !REFRESH;
     using FFTF = BlockFFT<T, NY, TPB, -1, 1, 1>;
     using FFTI = BlockFFT<T, NY, TPB, +1, 1, 1>;
     __shared__ cpx<T> lds[NY * TC];
@@ -476,6 +514,7 @@ __global__ __launch_bounds__(TPB* TC) PFC_WAVES(col_waves<T>()) void k_col_step(
     cpx<T>* L = lds + c * NY;
     const GBuf twy = gbuf(twy_);
     const GBuf ktab = gbuf(ktab_);
+    PFC_PROBE_AT(65536, 0);
     {
         const ColJob jb = col_job(blockIdx.x, c, TC, H);
         const int kx = jb.kx;
@@ -486,16 +525,6 @@ __global__ __launch_bounds__(TPB* TC) PFC_WAVES(col_waves<T>()) void k_col_step(
         const unsigned vi = lay_col_off(sI, H, jb.mode ? 0 : kx, t, E);
         const unsigned istep = lay_col_step(sI, H, TPB, E);
 
-        // state read by the update: Nh (loaded early when PREF)
-        cpx<T> S[U][PREF ? P : 1];
-        if constexpr (PREF) {
-#pragma unroll
-            for (int u = 0; u < U; u++) {
-                const GBuf src = gbuf(Nh + u * nstride);
-#pragma unroll
-                for (int q = 0; q < P; q++) S[u][q] = gload<T>(src, vn, q * TPB * E);
-            }
-        }
         cpx<T> M[U][P];
 #pragma unroll
         for (int u = 0; u < U; u++) {
@@ -503,7 +532,9 @@ __global__ __launch_bounds__(TPB* TC) PFC_WAVES(col_waves<T>()) void k_col_step(
 #pragma unroll
             for (int q = 0; q < P; q++) M[u][q] = col_in<T>(src, vr, q * rstep, jb.mode);
             FFTF::run(M[u], L, twy, t);
+            if (u == 0) PFC_PROBE_AT(65536, 1);
         }
+        PFC_PROBE_AT(65536, 2);
         if constexpr (REFRESH) {
             // N = FFT(n)/NxNy from the density's row transforms
             // (src/pfc-relax.c:1178-1184), written to Nh and read back below
@@ -525,26 +556,39 @@ __global__ __launch_bounds__(TPB* TC) PFC_WAVES(col_waves<T>()) void k_col_step(
         }
         if (jb.live) {
             const cpx<T> kxe = gload<T>(ktab, (unsigned)kx * E, 0u);
+            // The state loads of a batch of QB k points are issued together,
+            // ahead of the batch's stores (buffer loads are not moved past
+            // buffer stores).
+            constexpr int QB0 = sizeof(T) == 8 ? PFC_COL_QB_F64 : PFC_COL_QB_F32;
+            constexpr int QB = (QB0 < P) ? QB0 : P;
 #pragma unroll
-            for (int q = 0; q < P; q++) {
-                const cpx<T> kye = gload<T>(ktab, (unsigned)(H + 1 + t) * E, q * TPB * E);
-                const T k2 = kxe.x + kye.x;
-                const T G2 = kxe.y * kye.y;
-                cpx<T> Mk[U], Nk[U];
+            for (int q0 = 0; q0 < P; q0 += QB) {
+                cpx<T> Nb[QB][U], kyb[QB];
 #pragma unroll
-                for (int u = 0; u < U; u++) {
-                    Mk[u] = M[u][q];
-                    if constexpr (PREF) Nk[u] = S[u][q];
-                    else Nk[u] = gload<T>(gbuf(Nh + u * nstride), vn, q * TPB * E);
+                for (int i = 0; i < QB; i++) {
+                    kyb[i] = gload<T>(ktab, (unsigned)(H + 1 + t) * E, (q0 + i) * TPB * E);
+#pragma unroll
+                    for (int u = 0; u < U; u++)
+                        Nb[i][u] = gload<T>(gbuf(Nh + u * nstride), vn, (q0 + i) * TPB * E);
                 }
-                kupdate<U, T>(prm, k2, G2, Mk, Nk);
 #pragma unroll
-                for (int u = 0; u < U; u++) {
-                    gstore<T>(gbuf(Nh + u * nstride), vn, q * TPB * E, Nk[u]);
-                    M[u][q] = Nk[u];
+                for (int i = 0; i < QB; i++) {
+                    const int q = q0 + i;
+                    const T k2 = kxe.x + kyb[i].x;
+                    const T G2 = kxe.y * kyb[i].y;
+                    cpx<T> Mk[U];
+#pragma unroll
+                    for (int u = 0; u < U; u++) Mk[u] = M[u][q];
+                    kupdate<U, T>(prm, k2, G2, Mk, Nb[i]);
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        gstore<T>(gbuf(Nh + u * nstride), vn, q * TPB * E, Nb[i][u]);
+                        M[u][q] = Nb[i][u];
+                    }
                 }
             }
         }
+        PFC_PROBE_AT(65536, 3);
         if constexpr (sizeof(T) == 4) {
             // kx = 0 and kx = H: replace the stored state by its Hermitian part
             if (blockIdx.x == 0 || blockIdx.x == H / TC) {
@@ -569,6 +613,7 @@ __global__ __launch_bounds__(TPB* TC) PFC_WAVES(col_waves<T>()) void k_col_step(
             }
         }
     }
+    PFC_PROBE_AT(65536, 4);
 }
 
 // ------------------------------------------------ single-buffer row passes
