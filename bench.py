@@ -89,6 +89,7 @@ def accuracy_study(params, fields, Nx, Ny, dx, dy, nsteps, checkpoints):
     gp.set_fields(fields.astype(np.float32).astype(np.float64))
     t = 0
     rows = []
+    last = [c for c in checkpoints if c <= nsteps][-1]
     for c in checkpoints:
         if c > nsteps:
             break
@@ -102,17 +103,33 @@ def accuracy_study(params, fields, Nx, Ny, dx, dy, nsteps, checkpoints):
         nd = np.linalg.norm(xd)
         xm = gm.get_fields()
         em = float(np.linalg.norm(xm - xd) / nd)
-        del xm
         xp = gp.get_fields()
         ep = float(np.linalg.norm(xp - xd) / nd)
-        del xp, xd
-        rows.append({"step": c, "energy_per_area_rel": abs(fm - fd) / abs(fd),
-                     "density_rel_l2": em, "fave_fp64": fd,
-                     "fp64_from_fp32_rounded_start": {"energy_per_area_rel": abs(fp - fd) / abs(fd),
-                                                      "density_rel_l2": ep}})
+        row = {"step": c, "energy_per_area_rel": abs(fm - fd) / abs(fd),
+               "density_rel_l2": em, "fave_fp64": fd,
+               "fp64_from_fp32_rounded_start": {"energy_per_area_rel": abs(fp - fd) / abs(fd),
+                                                "density_rel_l2": ep}}
+        if c == last:
+            row["atoms"] = atom_positions(xd, xm, xp, dx, dy)
+        del xm, xp, xd
+        rows.append(row)
     for g in (gm, gd, gp):
         g.close()
     return rows
+
+
+def atom_positions(xd, xm, xp, dx, dy):
+    """Atoms of the summed density (the coordinator's conversion, lattice
+    constant 1) of the FP64 run, and the displacement of each mixed-run and
+    perturbed-run atom from its nearest FP64 atom, in lattice constants."""
+    from pfc_amd import LATTICE_A
+    from pfc_amd.coordinator import compare_atoms, coordinate
+    ad, box, nr, fl = coordinate(xd.sum(axis=0), dx, dy, LATTICE_A, 1.0)
+    am, _, _, _ = coordinate(xm.sum(axis=0), dx, dy, LATTICE_A, 1.0)
+    ap, _, _, _ = coordinate(xp.sum(axis=0), dx, dy, LATTICE_A, 1.0)
+    return {"field": "sum of the 3 densities", "unit": "lattice constants", "rings_fp64": nr,
+            "coordinator_flags_fp64": fl, "mixed": compare_atoms(ad, am, box),
+            "fp64_from_fp32_rounded_start": compare_atoms(ad, ap, box)}
 
 
 def cpu_baseline(params, fields, Nx, Ny, dx, dy, budget_s):

@@ -74,6 +74,24 @@ def build_host():
     _run(["gcc", "-O2", "-std=gnu99", "-Wall", "-Wextra", "-pthread", "-I", os.path.join(ROOT, "include"),
           *srcs, "-o", exe + ".tmp", "-L", LIBDIR, "-lpfcgpu", "-Wl,-rpath,$ORIGIN/../lib", "-lm"])
     os.replace(exe + ".tmp", exe)
+    build_coordinator()
+    return exe
+
+
+def build_coordinator():
+    """pfc-coordinator (coordinator.jar's command) and libpfccoord.so (the same
+    conversion for in-memory fields); plain C, no GPU.  -ffp-contract=off keeps
+    the arithmetic unfused, as Java evaluates it."""
+    os.makedirs(BINDIR, exist_ok=True)
+    os.makedirs(LIBDIR, exist_ok=True)
+    flags = ["-O2", "-std=gnu99", "-Wall", "-Wextra", "-ffp-contract=off"]
+    core = os.path.join(HOST, "pfc_coord.c")
+    exe = os.path.join(BINDIR, "pfc-coordinator")
+    _run(["gcc", *flags, os.path.join(HOST, "pfc_coordinator_main.c"), core, "-o", exe + ".tmp", "-lm"])
+    os.replace(exe + ".tmp", exe)
+    lib = os.path.join(LIBDIR, "libpfccoord.so")
+    _run(["gcc", *flags, "-shared", "-fPIC", core, "-o", lib + ".tmp", "-lm"])
+    os.replace(lib + ".tmp", lib)
     return exe
 
 
