@@ -125,6 +125,16 @@ int pfc_destroy(pfc_ctx* ctx);
 /* Human-readable text for a PFC_E* code. */
 const char* pfc_strerror(int code);
 
+/* Gaussian smoothing of one Ny x Nx field on the GPU: the spectrum is
+ * multiplied by exp(-k^2/(2 sigma^2)) (sigma > 0, in reciprocal units; dx, dy
+ * the grid spacings) and transformed back; in and out may be the same array.
+ * Replaces the computation of the reference's smooth tool
+ * (src/smooth.c:57-117; its final division by sum(C2R(G)), which is 1 up to
+ * round-off, is left out).  Uses a temporary context (PFC_FP64 or PFC_MIXED
+ * arithmetic; same size limits as pfc_create). */
+int pfc_smooth(const double* in, double* out, int Nx, int Ny, double dx, double dy,
+               double sigma, int precision);
+
 #ifdef __cplusplus
 }
 #endif

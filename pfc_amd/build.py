@@ -74,6 +74,11 @@ def build_host():
     _run(["gcc", "-O2", "-std=gnu99", "-Wall", "-Wextra", "-pthread", "-I", os.path.join(ROOT, "include"),
           *srcs, "-o", exe + ".tmp", "-L", LIBDIR, "-lpfcgpu", "-Wl,-rpath,$ORIGIN/../lib", "-lm"])
     os.replace(exe + ".tmp", exe)
+    sm = os.path.join(BINDIR, "pfc-smooth")
+    _run(["gcc", "-O2", "-std=gnu99", "-Wall", "-Wextra", "-pthread", "-I", os.path.join(ROOT, "include"),
+          os.path.join(HOST, "pfc_smooth_main.c"), os.path.join(HOST, "pfc_io.c"), "-o", sm + ".tmp",
+          "-L", LIBDIR, "-lpfcgpu", "-Wl,-rpath,$ORIGIN/../lib", "-lm"])
+    os.replace(sm + ".tmp", sm)
     build_coordinator()
     return exe
 

@@ -436,4 +436,54 @@ int pfc_nan_step(pfc_ctx* c, int* step) {
     return PFC_OK;
 }
 
+int pfc_smooth(const double* in, double* out, int Nx, int Ny, double dx, double dy,
+               double sigma, int precision) {
+    if (!in || !out || !(sigma > 0.0) || !(dx > 0.0) || !(dy > 0.0)) return PFC_EINVAL;
+    pfc_params p;
+    std::memset(&p, 0, sizeof p);
+    p.U = 1;
+    pfc_ctx* c = nullptr;
+    int rc = pfc_create(&c, &p, Nx, Ny, precision);
+    if (rc) return rc;
+    rc = pfc_set_box(c, dx, dy);
+    const size_t n = c->nr;
+    if (!rc) {
+        if (precision == PFC_FP64) {
+            if (hipMemcpyAsync(c->mreal, in, n * sizeof(double), hipMemcpyHostToDevice, c->stream) !=
+                hipSuccess)
+                rc = PFC_EDEVICE;
+            if (!rc) {
+                Launch<double>::row_r2c_any(c, c->mreal, c->I, c->sC, 1);
+                Launch<double>::col_smooth_any(c, c->I, c->O, sigma);
+                Launch<double>::row_c2r_any(c, c->O, c->sC, c->R, nullptr, 1);
+                if (hipGetLastError() != hipSuccess ||
+                    hipMemcpyAsync(out, c->R, n * sizeof(double), hipMemcpyDeviceToHost,
+                                   c->stream) != hipSuccess ||
+                    hipStreamSynchronize(c->stream) != hipSuccess)
+                    rc = PFC_EDEVICE;
+            }
+        } else {
+            std::vector<float> tmp(n);
+            for (size_t i = 0; i < n; i++) tmp[i] = (float)in[i];
+            if (hipMemcpyAsync(c->mreal, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice,
+                               c->stream) != hipSuccess)
+                rc = PFC_EDEVICE;
+            if (!rc) {
+                Launch<float>::row_r2c_any(c, c->mreal, c->I, c->sC, 1);
+                Launch<float>::col_smooth_any(c, c->I, c->O, sigma);
+                Launch<float>::row_c2r_any(c, c->O, c->sC, c->R, nullptr, 1);
+                if (hipGetLastError() != hipSuccess ||
+                    hipMemcpyAsync(tmp.data(), c->R, n * sizeof(float), hipMemcpyDeviceToHost,
+                                   c->stream) != hipSuccess ||
+                    hipStreamSynchronize(c->stream) != hipSuccess)
+                    rc = PFC_EDEVICE;
+            }
+            if (!rc)
+                for (size_t i = 0; i < n; i++) out[i] = tmp[i];
+        }
+    }
+    pfc_destroy(c);
+    return rc;
+}
+
 }  // extern "C"

@@ -723,6 +723,45 @@ __global__ __launch_bounds__(TPB* TC) void k_col_fwd(const cpx<T>* __restrict__ 
     }
 }
 
+// Gaussian smoothing along y for one packed row-spectral buffer (smooth.c:
+// src/smooth.c:82-103 multiplies the 2D spectrum by exp(-c k^2)/(W H)): src
+// holds R2C_x(f)/Nx; dst = IFFT_y(FFT_y(src) exp(-c k2)/Ny), so that the row
+// C2R of dst is the smoothed field.  The kx = 0 and kx = H columns (modes 1,
+// 2) keep only the real parts, as a 1D c2r does.
+template <typename T, int NY, int TPB, int TC>
+__global__ __launch_bounds__(TPB* TC) void k_col_smooth(const cpx<T>* __restrict__ src,
+                                                        cpx<T>* __restrict__ dst, int H, int sS,
+                                                        const cpx<T>* __restrict__ twy_,
+                                                        double dkx, double dky, double c) {
+    const GBuf twy = gbuf(twy_);
+    constexpr int P = NY / TPB;
+    constexpr unsigned E = sizeof(cpx<T>);
+    __shared__ cpx<T> lds[NY * TC];
+    const int cl = threadIdx.x / TPB;
+    const int t = threadIdx.x % TPB;
+    cpx<T>* L = lds + cl * NY;
+    const GBuf sb = gbuf(src), db = gbuf(dst);
+    const ColJob jb = col_job(blockIdx.x, cl, TC, H);
+    const unsigned vr = lay_col_off(sS, H, jb.mode ? 0 : jb.kx, t, E);
+    const unsigned rstep = lay_col_step(sS, H, TPB, E);
+    cpx<T> v[P];
+#pragma unroll
+    for (int q = 0; q < P; q++) v[q] = col_in<T>(sb, vr, q * rstep, jb.mode);
+    BlockFFT<T, NY, TPB, -1, 1, 1>::run(v, L, twy, t);
+    const double kx = jb.kx * dkx;
+    const double kx2 = kx * kx;
+#pragma unroll
+    for (int q = 0; q < P; q++) {
+        const double ky = ky_of(t + q * TPB, NY, dky);
+        v[q] = cscale(v[q], (T)(exp_cw(-c * (kx2 + ky * ky)) / NY));
+    }
+    BlockFFT<T, NY, TPB, +1, 1, 1>::run(v, L, twy, t);
+    if (jb.live) {
+#pragma unroll
+        for (int q = 0; q < P; q++) col_out<T>(db, vr, q * rstep, jb.mode, v[q]);
+    }
+}
+
 // Multiplier of one evaluate output at k2 (src/pfc-relax.c:1048-1077):
 // kind 0: P_u(k) (the multimode operator), 1: G(k) (Gaussian),
 // 2: (c2 - k2)^2 (gradient coupling, c2 = mu2 or the reference's nu2 alias).

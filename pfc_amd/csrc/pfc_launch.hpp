@@ -268,6 +268,28 @@ struct Launch {
         }
     }
     template <int NY>
+    static void col_smooth(pfc_ctx* c, const void* src, void* dst, double sigma) {
+        if constexpr (size_built(NY)) {
+            constexpr ColCfg g = col_cfg(NY);
+            const int H = c->Nx / 2;
+            hipLaunchKernelGGL((k_col_smooth<T, NY, g.tpb, g.tc>), dim3(H / g.tc + 1),
+                               dim3(g.tpb * g.tc), 0, c->stream, (const C*)src, (C*)dst, H, c->sC,
+                               (const C*)c->twy, c->hp.dkx, c->hp.dky, 0.5 / sigma / sigma);
+        }
+    }
+    static void col_smooth_any(pfc_ctx* c, const void* src, void* dst, double sigma) {
+        switch (c->Ny) {
+            case 64: col_smooth<64>(c, src, dst, sigma); break;
+            case 128: col_smooth<128>(c, src, dst, sigma); break;
+            case 256: col_smooth<256>(c, src, dst, sigma); break;
+            case 512: col_smooth<512>(c, src, dst, sigma); break;
+            case 1024: col_smooth<1024>(c, src, dst, sigma); break;
+            case 2048: col_smooth<2048>(c, src, dst, sigma); break;
+            case 4096: col_smooth<4096>(c, src, dst, sigma); break;
+            case 8192: col_smooth<8192>(c, src, dst, sigma); break;
+        }
+    }
+    template <int NY>
     static void col_eval(pfc_ctx* c) {
         if constexpr (size_built(NY)) {
             constexpr ColCfg g = col_cfg(NY);

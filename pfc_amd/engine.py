@@ -16,12 +16,13 @@ from .params import FP64, MIXED, PfcParams
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("PFC_LIB") or os.path.join(PKG, "lib", "libpfcgpu.so")
 BIN_PATH = os.path.join(PKG, "bin", "pfc-relax")
+SMOOTH_PATH = os.path.join(PKG, "bin", "pfc-smooth")
 
 # exported symbols of include/pfc_relax.h
 SYMBOLS = (
     "pfc_create", "pfc_set_fields", "pfc_set_box", "pfc_step", "pfc_evaluate",
     "pfc_get_fields", "pfc_get_energy", "pfc_nan_step", "pfc_timing", "pfc_timing_read",
-    "pfc_destroy", "pfc_strerror",
+    "pfc_destroy", "pfc_strerror", "pfc_smooth",
 )
 
 _lib = None
@@ -55,6 +56,8 @@ def load():
     L.pfc_timing.argtypes = [vp, ctypes.c_int]
     L.pfc_timing_read.argtypes = [vp, dp, ctypes.POINTER(ctypes.c_int)]
     L.pfc_destroy.argtypes = [vp]
+    L.pfc_smooth.argtypes = [dp, dp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                             ctypes.c_double, ctypes.c_int]
     L.pfc_strerror.argtypes = [ctypes.c_int]
     L.pfc_strerror.restype = ctypes.c_char_p
     for s in SYMBOLS:
@@ -151,5 +154,19 @@ class Engine:
         return s.value
 
 
-__all__ = ["Engine", "PfcError", "load", "strerror", "FP64", "MIXED", "LIB_PATH", "BIN_PATH",
+__all__ = ["Engine", "PfcError", "load", "strerror", "smooth", "FP64", "MIXED", "LIB_PATH", "BIN_PATH", "SMOOTH_PATH",
            "SYMBOLS"]
+
+
+def smooth(field, dx, dy, sigma, precision=FP64):
+    """Gaussian smoothing of one Ny x Nx field on the GPU (pfc_smooth)."""
+    L = load()
+    a = np.ascontiguousarray(field, dtype=np.float64)
+    Ny, Nx = a.shape
+    out = np.empty_like(a)
+    dp = ctypes.POINTER(ctypes.c_double)
+    rc = L.pfc_smooth(a.ctypes.data_as(dp), out.ctypes.data_as(dp), Nx, Ny, dx, dy, sigma,
+                      precision)
+    if rc != 0:
+        raise PfcError(rc, "pfc_smooth")
+    return out

@@ -441,11 +441,16 @@ static void* fmt_worker(void* arg) {
 
 int pfc_write_field(const char* path, const double* f, int Nx, int Ny, double dx, double dy,
                     int pwrite) {
+    return pfc_write_field_prec(path, f, Nx, Ny, dx, dy, pwrite, pwrite);
+}
+
+int pfc_write_field_prec(const char* path, const double* f, int Nx, int Ny, double dx, double dy,
+                         int phead, int pvalue) {
     FILE* out = fopen(path, "w");
     if (!out) return -1;
-    fprintf(out, "%d %d %.*lf %.*lf\n", Nx, Ny, pwrite, dx, pwrite, dy);
+    fprintf(out, "%d %d %.*lf %.*lf\n", Nx, Ny, phead, dx, phead, dy);
     const size_t n = (size_t)Nx * Ny;
-    const int prec = pwrite < 0 ? 6 : (pwrite > 300 ? 300 : pwrite);
+    const int prec = pvalue < 0 ? 6 : (pvalue > 300 ? 300 : pvalue);
     long nth = sysconf(_SC_NPROCESSORS_ONLN);
     if (nth < 1) nth = 1;
     if (nth > 16) nth = 16;

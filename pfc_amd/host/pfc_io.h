@@ -51,6 +51,11 @@ void pfc_step_name(const char* pattern, int t, char* out, size_t n);
 int pfc_write_field(const char* path, const double* f, int Nx, int Ny, double dx, double dy,
                     int pwrite);
 
+/* The same with separate precisions for the header (%.*lf) and the values
+ * (%.*e); smooth writes "%lf" and "%.12e" (src/smooth.c:123-130). */
+int pfc_write_field_prec(const char* path, const double* f, int Nx, int Ny, double dx, double dy,
+                         int phead, int pvalue);
+
 /* Log line of print_output (src/pfc-relax.c:861-897) to stdout and to the
  * ".out" file named after the input file. */
 void pfc_print_output(const run_cfg* cfg, int t, int elapsed, double dx, double dy, double fave,
