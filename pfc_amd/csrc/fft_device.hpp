@@ -187,6 +187,18 @@ __device__ __forceinline__ void gstore<float>(GBuf b, unsigned voff, unsigned so
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V, x), b.r, (int)voff, (int)soff, 0);
 }
 
+// one real value
+template <typename T>
+__device__ __forceinline__ T gload_real(GBuf b, unsigned voff);
+template <>
+__device__ __forceinline__ double gload_real<double>(GBuf b, unsigned voff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(b.r, (int)voff, 0, 0));
+}
+template <>
+__device__ __forceinline__ float gload_real<float>(GBuf b, unsigned voff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(b.r, (int)voff, 0, 0));
+}
+
 // one real value (the real or imaginary half of a complex element)
 template <typename T>
 __device__ __forceinline__ void gstore_real(GBuf b, unsigned voff, unsigned soff, T x);
@@ -214,15 +226,24 @@ struct BlockFFT {
     static constexpr int S = ilog(P, L);          // number of radix-P stages
     static constexpr int R0 = L / ipow(P, S);     // leading stage radix (1 if none)
 
+    // Stage twiddle e^{-2 pi i m/L} (conjugated for the inverse): entry m*TWS
+    // of a table of L*TWS entries, in global memory or copied to LDS.  Global and LDS reads are
+    // counted by different wait counters, so LDS twiddles never wait for
+    // the global loads and stores in flight before them.
     __device__ static __forceinline__ cpx<T> twid(GBuf tw, int m, unsigned z) {
         cpx<T> w = gload<T>(tw, (unsigned)(m * TWS) * sizeof(cpx<T>), z);
         if constexpr (DIR > 0) w.y = -w.y;
         return w;
     }
+    __device__ static __forceinline__ cpx<T> twid(const cpx<T>* ltw, int m, unsigned) {
+        cpx<T> w = ltw[m * TWS];
+        if constexpr (DIR > 0) w.y = -w.y;
+        return w;
+    }
 
     // One radix-R Stockham stage on butterflies j = t + b*TPB, inputs in in[].
-    template <int R, bool LAST>
-    __device__ static __forceinline__ void stage(cpx<T> (&in)[P], int Ns, cpx<T>* lds, GBuf tw,
+    template <int R, bool LAST, typename TW>
+    __device__ static __forceinline__ void stage(cpx<T> (&in)[P], int Ns, cpx<T>* lds, TW tw,
                                                  int t, unsigned z) {
         constexpr int B = P / R;
 #pragma unroll
@@ -295,7 +316,8 @@ struct BlockFFT {
     //
     // The twiddle loads take an opaque zero offset, so that the compiler does
     // not share them (and keep them live) across the transforms of a kernel.
-    __device__ static __forceinline__ void run(cpx<T> (&v)[P], cpx<T>* lds, GBuf tw, int t) {
+    template <typename TW>
+    __device__ static __forceinline__ void run(cpx<T> (&v)[P], cpx<T>* lds, TW tw, int t) {
         int Ns = 1;
         unsigned z = 0;
         asm volatile("" : "+s"(z));

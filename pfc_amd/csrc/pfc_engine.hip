@@ -8,7 +8,7 @@
 //   mreal         U*nr real    densities in real space (materialised on demand)
 //   R             (2U + U(U-1)/2)*nr real  evaluate's real-space outputs
 //   f             nr doubles   energy density of the last evaluate
-//   ktab          (Nx/2+1 + Ny) E  {k^2, exp(-k^2/sigma^2)} per kx, then per ky
+//   ktab          Gaussian factors exp(-k^2/sigma^2), per kx then per ky (ktab_gy)
 // Everything runs on one HIP stream owned by the context.
 #include <hip/hip_runtime.h>
 
@@ -91,24 +91,28 @@ void fill_devparams(pfc_ctx* c) {
     }
 }
 
-// k tables of the column pass: for kx = 0..Nx/2 and then ky (j < Ny/2: j dky,
-// else (j - Ny) dky) the pair {k^2, exp(2 a k^2)} with a = -0.5/sigma^2, so
-// that k2 = kx^2 + ky^2 rounds as in the reference (src/pfc-relax.c:1217-1225)
-// and G2 = exp(a k2)^2 is the product of the two second entries.
+// Gaussian factor table of the column pass (layout at ktab_gy): for kx =
+// 0..Nx/2 and then per row index j (ky = j dky for j < Ny/2, else (j - Ny)
+// dky) the factor exp(2 a k^2) with a = -0.5/sigma^2, so that G2 = exp(a k2)^2
+// (src/pfc-relax.c:1223-1225) is the product of a kx and a ky factor.
+size_t ktab_bytes(const pfc_ctx* c) {
+    return (size_t)(ktab_gy(c->Nx / 2) + c->Ny) * (c->esz / 2);
+}
+
 template <typename T>
 void fill_ktab(pfc_ctx* c) {
     const int H = c->Nx / 2;
-    cpx<T>* k = (cpx<T>*)c->hktab;
+    T* k = (T*)c->hktab;
     const double a2 = 2.0 * c->hp.sig_a;
+    std::memset(c->hktab, 0, (size_t)(ktab_gy(H) + c->Ny) * sizeof(T));
     for (int i = 0; i <= H; i++) {
         double kx = i * c->hp.dkx;
         kx *= kx;
-        k[i] = {(T)kx, (T)std::exp(a2 * kx)};
+        k[i] = (T)std::exp(a2 * kx);
     }
     for (int j = 0; j < c->Ny; j++) {
         const double ky = (j < c->Ny / 2) ? j * c->hp.dky : (j - c->Ny) * c->hp.dky;
-        const double ky2 = ky * ky;
-        k[H + 1 + j] = {(T)ky2, (T)std::exp(a2 * ky2)};
+        k[ktab_gy(H) + j] = (T)std::exp(a2 * (ky * ky));
     }
 }
 
@@ -118,8 +122,7 @@ int upload_params(pfc_ctx* c) {
     if (c->prec == PFC_FP64) fill_ktab<double>(c);
     else fill_ktab<float>(c);
     HIPCHK(hipMemcpyAsync(c->dprm, &c->hp, sizeof(DevParams), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(c->ktab, c->hktab, (size_t)(c->Nx / 2 + 1 + c->Ny) * c->esz,
-                          hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->ktab, c->hktab, ktab_bytes(c), hipMemcpyHostToDevice, c->stream));
     return PFC_OK;
 }
 
@@ -242,8 +245,8 @@ int pfc_create(pfc_ctx** out, const pfc_params* p, int Nx, int Ny, int precision
 #endif
     rc |= dalloc(&c->twx, Nx * c->esz);
     rc |= dalloc(&c->twy, Ny * c->esz);
-    rc |= dalloc(&c->ktab, (size_t)(Nx / 2 + 1 + Ny) * c->esz);
-    c->hktab = malloc((size_t)(Nx / 2 + 1 + Ny) * c->esz);
+    rc |= dalloc(&c->ktab, ktab_bytes(c));
+    c->hktab = malloc(ktab_bytes(c));
     if (!c->hktab) rc |= PFC_ENOMEM;
     if (rc != PFC_OK) {
         free_all(c);

@@ -87,6 +87,11 @@ struct DevParams {
     } while (0)
 #endif
 
+// Gaussian factor table of the column pass (per precision, set by
+// pfc_set_box): exp(-kx^2/sigma^2) for kx = 0..H at [0, H], then
+// exp(-ky^2/sigma^2) for the Ny row indices from ktab_gy(H), 16-byte aligned.
+__host__ __device__ constexpr int ktab_gy(int H) { return (H + 1 + 3) & ~3; }
+
 // Blocks are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8).
 // Remap so each XCD walks a contiguous range of logical indices: neighbouring
 // rows then share that XCD's L2, which merges their partial cache lines.
@@ -261,15 +266,20 @@ __device__ __forceinline__ void kupdate(const DevParams* __restrict__ p, T k2, T
 // C2R pre-processing of one row held in natural order (thread t holds
 // X[t + q*TPB], X[0] packed as (DC, Nyquist)): produces Z with
 // IFFT_H(Z)[n] = x[2n] + i x[2n+1], x the unnormalised c2r of X.
-template <typename T, int H, int TPB, int LS>
-__device__ __forceinline__ void c2r_pre(cpx<T> (&v)[H / TPB], cpx<T>* lds, GBuf twx, int t) {
-    constexpr int P = H / TPB;
-    // e^{-2 pi i k/Nx} for k = t + q*TPB is e^{-2 pi i t/Nx} e^{-2 pi i q/(2P)};
-    // loaded through an opaque zero offset so that the compiler recomputes
-    // the products per call instead of keeping them live across the kernel
+// wt = e^{-2 pi i t/Nx}: e^{-2 pi i k/Nx} for k = t + q*TPB is wt e^{-2 pi i q/(2P)}.
+// row_wt loads it through an opaque zero offset, so that a kernel calling the
+// helpers several times reloads it rather than keeping it (and the products
+// with the roots) live throughout.
+template <typename T>
+__device__ __forceinline__ cpx<T> row_wt(GBuf twx, int t) {
     unsigned z = 0;
     asm volatile("" : "+s"(z));
-    const cpx<T> wt = gload<T>(twx, (unsigned)t * sizeof(cpx<T>), z);
+    return gload<T>(twx, (unsigned)t * sizeof(cpx<T>), z);
+}
+
+template <typename T, int H, int TPB, int LS>
+__device__ __forceinline__ void c2r_pre(cpx<T> (&v)[H / TPB], cpx<T>* lds, cpx<T> wt, int t) {
+    constexpr int P = H / TPB;
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < P; q++) lds[(t + q * TPB) * LS] = v[q];
@@ -294,11 +304,8 @@ __device__ __forceinline__ void c2r_pre(cpx<T> (&v)[H / TPB], cpx<T>* lds, GBuf 
 // R2C post-processing: v holds Z = FFT_H(z), z[n] = x[2n] + i x[2n+1], in
 // natural order; produces X[k] (k < H) with X[0] packed as (DC, Nyquist).
 template <typename T, int H, int TPB, int LS>
-__device__ __forceinline__ void r2c_post(cpx<T> (&v)[H / TPB], cpx<T>* lds, GBuf twx, int t) {
+__device__ __forceinline__ void r2c_post(cpx<T> (&v)[H / TPB], cpx<T>* lds, cpx<T> wt, int t) {
     constexpr int P = H / TPB;
-    unsigned z = 0;
-    asm volatile("" : "+s"(z));
-    const cpx<T> wt = gload<T>(twx, (unsigned)t * sizeof(cpx<T>), z);
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < P; q++) lds[(t + q * TPB) * LS] = v[q];
@@ -318,6 +325,29 @@ __device__ __forceinline__ void r2c_post(cpx<T> (&v)[H / TPB], cpx<T>* lds, GBuf
             const cpx<T> w = cmul(wt, root<T>(q, 2 * P, -1));   // e^{-2 pi i k/Nx}
             const cpx<T> r = cadd(s, cmul(xo2, w));
             v[q] = cscale(r, (T)0.5);
+        }
+    }
+}
+
+// Copies NBYTES (a multiple of 16) from a global buffer at byte offset src
+// into LDS with asynchronous buffer-to-LDS loads, 16 bytes per lane, no
+// registers involved.  Completion is counted like a global load, so issue
+// the copy before the field loads; it is complete once those are waited
+// for, and a barrier then makes it visible to the block.  Reading tables
+// from LDS afterwards never waits behind global loads and stores, which
+// complete in issue order under one counter.
+template <int NBYTES, int NT>
+__device__ __forceinline__ void lds_copy16(GBuf g, unsigned src, void* dst, int tid) {
+    static_assert(NBYTES % 16 == 0, "16-byte granules");
+    constexpr int WAVES = NT / 64;
+    const int wave = tid / 64, lane = tid % 64;
+#pragma unroll
+    for (int base = 0; base < NBYTES; base += WAVES * 1024) {
+        const int off = base + wave * 1024;   // this wave's 64 x 16-byte chunk
+        if (off < NBYTES && off + lane * 16 < NBYTES) {
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                g.r, (__attribute__((address_space(3))) void*)((char*)dst + off), 16,
+                (unsigned)(src + off + lane * 16), 0, 0, 0);
         }
     }
 }
@@ -383,7 +413,7 @@ __global__ __launch_bounds__(TPB* TR) PFC_WAVES(row_waves<T>()) void k_row_step(
     }
 #pragma unroll
     for (int u = 0; u < U; u++) {
-        c2r_pre<T, H, TPB, TR>(m[u], L, twx, t);
+        c2r_pre<T, H, TPB, TR>(m[u], L, row_wt<T>(twx, t), t);
         FFTI::run(m[u], L, twx, t);
         if (u == 0) PFC_PROBE_AT(0, 1);
     }
@@ -401,7 +431,7 @@ __global__ __launch_bounds__(TPB* TR) PFC_WAVES(row_waves<T>()) void k_row_step(
 #pragma unroll
             for (int q = 0; q < P; q++) w[q] = m[u][q];
             FFTF::run(w, L, twx, t);
-            r2c_post<T, H, TPB, TR>(w, L, twx, t);
+            r2c_post<T, H, TPB, TR>(w, L, row_wt<T>(twx, t), t);
             const GBuf dst = gbuf(K + u * fstride);
 #pragma unroll
             for (int q = 0; q < P; q++) gstore<T>(dst, voff, q * qstep, w[q]);
@@ -424,7 +454,7 @@ __global__ __launch_bounds__(TPB* TR) PFC_WAVES(row_waves<T>()) void k_row_step(
 #pragma unroll
     for (int u = 0; u < U; u++) {
         FFTF::run(m[u], L, twx, t);
-        r2c_post<T, H, TPB, TR>(m[u], L, twx, t);
+        r2c_post<T, H, TPB, TR>(m[u], L, row_wt<T>(twx, t), t);
         const GBuf dst = gbuf(J + u * fstride);
 #pragma unroll
         for (int q = 0; q < P; q++) gstore<T>(dst, voff, q * qstep, m[u][q]);
@@ -490,8 +520,8 @@ __device__ __forceinline__ void hermitian_cols(cpx<T> (&M)[U][NY / TPB], cpx<T>*
 // Column pass of a step: TC columns per block (lane = column-major within the
 // block), each column all Ny rows; the extra last block runs kx = H.
 // fstride: row-spectral field stride (H*Ny), nstride: state stride ((H+1)*Ny).
-// ktab: H+1 entries {kx^2, exp(-kx^2/sigma^2)} then Ny entries for ky, so
-// k2 = kx^2 + ky^2 rounds as in the reference and G2 is their product.
+// ktab: the Gaussian factors (ktab_gy); G2 is the product of the kx and the
+// ky factor.
 #ifndef PFC_COL_QB_F64
 #define PFC_COL_QB_F64 4
 #endif
@@ -502,15 +532,21 @@ template <typename T, int NY, int TPB, int TC, int U, bool REFRESH>
 __global__ __launch_bounds__(TPB* TC) PFC_WAVES(col_waves<T>()) void k_col_step(
     const cpx<T>* __restrict__ J, const cpx<T>* __restrict__ K, cpx<T>* __restrict__ Nh,
     cpx<T>* __restrict__ I, size_t fstride, size_t nstride, int H, int sI, int sJ,
-    const cpx<T>* __restrict__ twy_, const cpx<T>* __restrict__ ktab_,
+    const cpx<T>* __restrict__ twy_, const T* __restrict__ ktab_,
     const DevParams* __restrict__ prm) {
     constexpr int P = NY / TPB;
     constexpr unsigned E = sizeof(cpx<T>);
     using FFTF = BlockFFT<T, NY, TPB, -1, 1, 1>;
     using FFTI = BlockFFT<T, NY, TPB, +1, 1, 1>;
+    // twiddle and ky-factor tables in LDS where they fit beside the
+    // exchange buffer (all sizes but FP64 Ny = 8192)
+    constexpr bool LTAB = NY * TC * E + NY * E + NY * sizeof(T) <= 160 * 1024;
     __shared__ cpx<T> lds[NY * TC];
-    const int c = threadIdx.x / TPB;
-    const int t = threadIdx.x % TPB;
+    __shared__ __attribute__((aligned(16))) cpx<T> ltw[LTAB ? NY : 1];   // e^{-2 pi i m/Ny}
+    __shared__ __attribute__((aligned(16))) T lgy[LTAB ? NY : 1];        // exp(-ky^2/sigma^2)
+    const int tid = threadIdx.x;
+    const int c = tid / TPB;
+    const int t = tid % TPB;
     cpx<T>* L = lds + c * NY;
     const GBuf twy = gbuf(twy_);
     const GBuf ktab = gbuf(ktab_);
@@ -525,13 +561,33 @@ __global__ __launch_bounds__(TPB* TC) PFC_WAVES(col_waves<T>()) void k_col_step(
         const unsigned vi = lay_col_off(sI, H, jb.mode ? 0 : kx, t, E);
         const unsigned istep = lay_col_step(sI, H, TPB, E);
 
+        // tables first, then all fields' J
+        if constexpr (LTAB) {
+            lds_copy16<NY * (int)E, TPB * TC>(twy, 0u, ltw, tid);
+            lds_copy16<NY * (int)sizeof(T), TPB * TC>(ktab, (unsigned)ktab_gy(H) * sizeof(T), lgy,
+                                                       tid);
+        }
+        const T gx = gload_real<T>(ktab, (unsigned)kx * sizeof(T));
+        __builtin_amdgcn_sched_barrier(0);
         cpx<T> M[U][P];
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const GBuf src = gbuf(J + u * fstride);
 #pragma unroll
             for (int q = 0; q < P; q++) M[u][q] = col_in<T>(src, vr, q * rstep, jb.mode);
-            FFTF::run(M[u], L, twy, t);
+        }
+        __syncthreads();   // the table copies are complete once the loads before them are
+        auto fwd = [&](cpx<T>(&v)[P]) {
+            if constexpr (LTAB) FFTF::run(v, L, (const cpx<T>*)ltw, t);
+            else FFTF::run(v, L, twy, t);
+        };
+        auto inv = [&](cpx<T>(&v)[P]) {
+            if constexpr (LTAB) FFTI::run(v, L, (const cpx<T>*)ltw, t);
+            else FFTI::run(v, L, twy, t);
+        };
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            fwd(M[u]);
             if (u == 0) PFC_PROBE_AT(65536, 1);
         }
         PFC_PROBE_AT(65536, 2);
@@ -547,7 +603,7 @@ __global__ __launch_bounds__(TPB* TC) PFC_WAVES(col_waves<T>()) void k_col_step(
                 cpx<T> w[P];
 #pragma unroll
                 for (int q = 0; q < P; q++) w[q] = col_in<T>(src, vr, q * rstep, jb.mode);
-                FFTF::run(w, L, twy, t);
+                fwd(w);
                 if (jb.live) {
 #pragma unroll
                     for (int q = 0; q < P; q++) gstore<T>(dst, vn, q * TPB * E, cscale(w[q], sc));
@@ -555,18 +611,18 @@ __global__ __launch_bounds__(TPB* TC) PFC_WAVES(col_waves<T>()) void k_col_step(
             }
         }
         if (jb.live) {
-            const cpx<T> kxe = gload<T>(ktab, (unsigned)kx * E, 0u);
-            // The state loads of a batch of QB k points are issued together,
-            // ahead of the batch's stores (buffer loads are not moved past
-            // buffer stores).
+            // k2 = kx^2 + ky^2 rounded as in the reference (src/pfc-relax.c:1217-1222)
+            const double kxd = kx * prm->dkx;
+            const double kx2 = kxd * kxd;
+            // State loads in batches of QB k points, each batch's loads issued
+            // together (a load issued after a store completes after it).
             constexpr int QB0 = sizeof(T) == 8 ? PFC_COL_QB_F64 : PFC_COL_QB_F32;
             constexpr int QB = (QB0 < P) ? QB0 : P;
 #pragma unroll
             for (int q0 = 0; q0 < P; q0 += QB) {
-                cpx<T> Nb[QB][U], kyb[QB];
+                cpx<T> Nb[QB][U];
 #pragma unroll
                 for (int i = 0; i < QB; i++) {
-                    kyb[i] = gload<T>(ktab, (unsigned)(H + 1 + t) * E, (q0 + i) * TPB * E);
 #pragma unroll
                     for (int u = 0; u < U; u++)
                         Nb[i][u] = gload<T>(gbuf(Nh + u * nstride), vn, (q0 + i) * TPB * E);
@@ -574,16 +630,21 @@ __global__ __launch_bounds__(TPB* TC) PFC_WAVES(col_waves<T>()) void k_col_step(
 #pragma unroll
                 for (int i = 0; i < QB; i++) {
                     const int q = q0 + i;
-                    const T k2 = kxe.x + kyb[i].x;
-                    const T G2 = kxe.y * kyb[i].y;
+                    const int j = t + q * TPB;
+                    const double ky = ky_of(j, NY, prm->dky);
+                    const T k2 = (T)(kx2 + ky * ky);
+                    T gy;
+                    if constexpr (LTAB) gy = lgy[j];
+                    else gy = gload_real<T>(ktab, (unsigned)(ktab_gy(H) + j) * sizeof(T));
+                    const T G2 = gx * gy;
                     cpx<T> Mk[U];
 #pragma unroll
                     for (int u = 0; u < U; u++) Mk[u] = M[u][q];
                     kupdate<U, T>(prm, k2, G2, Mk, Nb[i]);
 #pragma unroll
                     for (int u = 0; u < U; u++) {
-                        gstore<T>(gbuf(Nh + u * nstride), vn, q * TPB * E, Nb[i][u]);
                         M[u][q] = Nb[i][u];
+                        gstore<T>(gbuf(Nh + u * nstride), vn, q * TPB * E, M[u][q]);
                     }
                 }
             }
@@ -605,7 +666,7 @@ __global__ __launch_bounds__(TPB* TC) PFC_WAVES(col_waves<T>()) void k_col_step(
         }
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            FFTI::run(M[u], L, twy, t);
+            inv(M[u]);
             if (jb.live) {
                 const GBuf dst = gbuf(I + u * fstride);
 #pragma unroll
@@ -643,7 +704,8 @@ __global__ __launch_bounds__(TPB* TR) void k_row_c2r(const cpx<T>* __restrict__ 
     const GBuf sb = gbuf(src + b * fstride);
 #pragma unroll
     for (int q = 0; q < P; q++) v[q] = gload<T>(sb, soff, q * sstep);
-    c2r_pre<T, H, TPB, TR>(v, L, twx, t);
+    const cpx<T> wt = gload<T>(twx, (unsigned)t * E, 0u);
+    c2r_pre<T, H, TPB, TR>(v, L, wt, t);
     BlockFFT<T, H, TPB, +1, 2, TR>::run(v, L, twx, t);
     const GBuf db = gbuf(dst + b * rstride);
     const unsigned roff = ((unsigned)y * NX / 2 + (unsigned)t) * E;
@@ -651,7 +713,7 @@ __global__ __launch_bounds__(TPB* TR) void k_row_c2r(const cpx<T>* __restrict__ 
     for (int q = 0; q < P; q++) gstore<T>(db, roff, q * TPB * E, v[q]);
     if (K) {
         BlockFFT<T, H, TPB, -1, 2, TR>::run(v, L, twx, t);
-        r2c_post<T, H, TPB, TR>(v, L, twx, t);
+        r2c_post<T, H, TPB, TR>(v, L, wt, t);
         const GBuf kb = gbuf(K + b * fstride);
 #pragma unroll
         for (int q = 0; q < P; q++) gstore<T>(kb, voff, q * qstep, v[q]);
@@ -682,7 +744,7 @@ __global__ __launch_bounds__(TPB* TR) void k_row_r2c(const T* __restrict__ src,
 #pragma unroll
     for (int q = 0; q < P; q++) v[q] = gload<T>(sb, roff, q * TPB * E);
     BlockFFT<T, H, TPB, -1, 2, TR>::run(v, L, twx, t);
-    r2c_post<T, H, TPB, TR>(v, L, twx, t);
+    r2c_post<T, H, TPB, TR>(v, L, gload<T>(twx, (unsigned)t * E, 0u), t);
     const GBuf db = gbuf(dst + b * fstride);
     const unsigned voff = lay_row_off(sD, H, t, y, E), qstep = lay_row_step(sD, TPB, E);
 #pragma unroll

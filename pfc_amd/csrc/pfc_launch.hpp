@@ -108,7 +108,7 @@ struct pfc_ctx {
     pfc::DevParams hp;
     pfc::DevParams* dprm;
     void *I, *J, *K, *Nh, *O, *mreal, *R, *twx, *twy;
-    void* ktab;   // (Nx/2+1) + Ny entries {k^2, exp(-k^2/sigma^2)}, set by pfc_set_box
+    void* ktab;   // Gaussian factors exp(-k^2/sigma^2) per kx and per ky (ktab_gy), set by pfc_set_box
     void* hktab;  // host staging of ktab
     double *f, *partials, *sums;
     int* nan_flag;
@@ -173,7 +173,7 @@ struct Launch {
             hipLaunchKernelGGL((k_col_step<T, NY, g.tpb, g.tc, U, REF>), dim3(H / g.tc + 1),
                                dim3(g.tpb * g.tc), 0, c->stream, (const C*)c->J, (const C*)c->K,
                                (C*)c->Nh, (C*)c->I, c->nk, c->nkn, H, c->sI, c->sJ, (const C*)c->twy,
-                               (const C*)c->ktab, c->dprm);
+                               (const T*)c->ktab, c->dprm);
         }
     }
     template <int NY, int U>
