@@ -80,6 +80,7 @@ def build_host():
           "-L", LIBDIR, "-lpfcgpu", "-Wl,-rpath,$ORIGIN/../lib", "-lm"])
     os.replace(sm + ".tmp", sm)
     build_coordinator()
+    build_init()
     return exe
 
 
@@ -97,6 +98,18 @@ def build_coordinator():
     lib = os.path.join(LIBDIR, "libpfccoord.so")
     _run(["gcc", *flags, "-shared", "-fPIC", core, "-o", lib + ".tmp", "-lm"])
     os.replace(lib + ".tmp", lib)
+    return exe
+
+
+def build_init():
+    """pfc-init (pfc-init.jar's command): plain C, no GPU, unfused arithmetic
+    so that grain values round as Java's do."""
+    os.makedirs(BINDIR, exist_ok=True)
+    exe = os.path.join(BINDIR, "pfc-init")
+    _run(["gcc", "-O2", "-std=gnu99", "-Wall", "-Wextra", "-ffp-contract=off", "-pthread",
+          "-I", os.path.join(ROOT, "include"), os.path.join(HOST, "pfc_init_main.c"), os.path.join(HOST, "pfc_jrand.c"),
+          os.path.join(HOST, "pfc_io.c"), "-o", exe + ".tmp", "-lm"])
+    os.replace(exe + ".tmp", exe)
     return exe
 
 

@@ -449,7 +449,11 @@ int pfc_write_field_prec(const char* path, const double* f, int Nx, int Ny, doub
     FILE* out = fopen(path, "w");
     if (!out) return -1;
     fprintf(out, "%d %d %.*lf %.*lf\n", Nx, Ny, phead, dx, phead, dy);
-    const size_t n = (size_t)Nx * Ny;
+    pfc_write_values(out, f, (size_t)Nx * Ny, pvalue);
+    return fclose(out) == 0 ? 0 : -1;
+}
+
+void pfc_write_values(FILE* out, const double* f, size_t n, int pvalue) {
     const int prec = pvalue < 0 ? 6 : (pvalue > 300 ? 300 : pvalue);
     long nth = sysconf(_SC_NPROCESSORS_ONLN);
     if (nth < 1) nth = 1;
@@ -477,7 +481,6 @@ int pfc_write_field_prec(const char* path, const double* f, int Nx, int Ny, doub
             free(jobs[k].buf);
         }
     }
-    return fclose(out) == 0 ? 0 : -1;
 }
 
 void pfc_print_output(const run_cfg* cfg, int t, int elapsed, double dx, double dy, double fave,

@@ -56,6 +56,9 @@ int pfc_write_field(const char* path, const double* f, int Nx, int Ny, double dx
 int pfc_write_field_prec(const char* path, const double* f, int Nx, int Ny, double dx, double dy,
                          int phead, int pvalue);
 
+/* The value lines alone: one "%.*e" per value, formatted in parallel. */
+void pfc_write_values(FILE* out, const double* f, size_t n, int pvalue);
+
 /* Log line of print_output (src/pfc-relax.c:861-897) to stdout and to the
  * ".out" file named after the input file. */
 void pfc_print_output(const run_cfg* cfg, int t, int elapsed, double dx, double dy, double fave,
