@@ -1,8 +1,8 @@
 // pfc_engine.hip -- libpfcgpu: the C ABI of include/pfc_relax.h on gfx950.
 //
 // Device state of a context (E = complex element, FP64 or FP32 by precision;
-// nk = (Nx/2)*Ny, nr = Nx*Ny):
-//   I, J, K       U*nk E each, nk = (Nx/2)*Ny (packed row transforms)
+// nk = (Nx/2)*P with column pitch P = Ny (+ padding from Ny = 1024), nr = Nx*Ny):
+//   I, J, K       U*nk E each (packed row transforms)
 //   Nh            U*nkn E, nkn = (Nx/2+1)*Ny (the spectral state)
 //   O             (2U + U(U-1)/2)*nk E   evaluate's k-space outputs
 //   mreal         U*nr real    densities in real space (materialised on demand)
@@ -209,7 +209,7 @@ int pfc_create(pfc_ctx** out, const pfc_params* p, int Nx, int Ny, int precision
     c->prec = precision;
     c->p = *p;
     c->nO = 2 * c->U + c->U * (c->U - 1) / 2;
-    c->nk = (size_t)(Nx / 2) * Ny;
+    c->nk = (size_t)(Nx / 2) * lay_pitch(ilog2(Ny), precision == PFC_FP64 ? 16u : 8u);   // column-major, padded columns
     c->nkn = (size_t)(Nx / 2 + 1) * Ny;   // half spectrum, kx = 0..Nx/2
     c->nr = (size_t)Nx * Ny;
     c->esz = (precision == PFC_FP64) ? 16 : 8;
