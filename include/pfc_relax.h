@@ -135,6 +135,20 @@ const char* pfc_strerror(int code);
 int pfc_smooth(const double* in, double* out, int Nx, int Ny, double dx, double dy,
                double sigma, int precision);
 
+/* Orientation field, and optionally deformation field, of one H x W density
+ * (row-major), the computation of the reference's fields tool
+ * (src/fields.c:104-359, driven by main :368-424) on the GPU in FP64.
+ *   phi: 2*W*H doubles, (re, im) per point, row-major, divided by max |phi|
+ *        (fields.c:104-226);
+ *   chi: NULL, or W*H doubles: the sum over scales n = 2, 4, 8, ... while
+ *        n*4*pi/sqrt(3) < min(W dx, H dy)/2 of |grad phi|^power smoothed by a
+ *        Gaussian of width 1/n, each divided by its maximum (fields.c:229-359).
+ * fold: order of rotational symmetry (even); sk: width of the complex kernel
+ * around |k| = 1; sb: width of the Gaussian smoothing of phi (reciprocal
+ * units).  W and H: powers of two in [64, 8192]. */
+int pfc_fields(const double* density, int W, int H, double dx, double dy, int fold, double sk,
+               double sb, double power, double* phi, double* chi);
+
 #ifdef __cplusplus
 }
 #endif

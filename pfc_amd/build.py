@@ -4,7 +4,7 @@
     python -m pfc_amd.build --lib      # only the HIP library
 
 Outputs: pfc_amd/lib/libpfcgpu.so and pfc_amd/bin/pfc-relax (rpath to ../lib).
-The three HIP translation units (FP64 kernels, FP32 kernels, engine) compile
+The HIP translation units (FP64 kernels, FP32 kernels, engine, fields) compile
 in parallel.
 """
 from __future__ import annotations
@@ -32,7 +32,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-pass-failed",
 ]
-HIP_UNITS = ["pfc_launch_f64.hip", "pfc_launch_f32.hip", "pfc_engine.hip"]
+HIP_UNITS = ["pfc_launch_f64.hip", "pfc_launch_f32.hip", "pfc_engine.hip", "pfc_fields.hip"]
 HOST_SRCS = ["pfc_relax_main.c", "pfc_io.c"]
 
 
@@ -79,6 +79,11 @@ def build_host():
           os.path.join(HOST, "pfc_smooth_main.c"), os.path.join(HOST, "pfc_io.c"), "-o", sm + ".tmp",
           "-L", LIBDIR, "-lpfcgpu", "-Wl,-rpath,$ORIGIN/../lib", "-lm"])
     os.replace(sm + ".tmp", sm)
+    fl = os.path.join(BINDIR, "pfc-fields")
+    _run(["gcc", "-O2", "-std=gnu99", "-Wall", "-Wextra", "-I", os.path.join(ROOT, "include"),
+          os.path.join(HOST, "pfc_fields_main.c"), "-o", fl + ".tmp",
+          "-L", LIBDIR, "-lpfcgpu", "-Wl,-rpath,$ORIGIN/../lib", "-lm"])
+    os.replace(fl + ".tmp", fl)
     build_coordinator()
     build_init()
     return exe

@@ -17,12 +17,13 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("PFC_LIB") or os.path.join(PKG, "lib", "libpfcgpu.so")
 BIN_PATH = os.path.join(PKG, "bin", "pfc-relax")
 SMOOTH_PATH = os.path.join(PKG, "bin", "pfc-smooth")
+FIELDS_PATH = os.path.join(PKG, "bin", "pfc-fields")
 
 # exported symbols of include/pfc_relax.h
 SYMBOLS = (
     "pfc_create", "pfc_set_fields", "pfc_set_box", "pfc_step", "pfc_evaluate",
     "pfc_get_fields", "pfc_get_energy", "pfc_nan_step", "pfc_timing", "pfc_timing_read",
-    "pfc_destroy", "pfc_strerror", "pfc_smooth",
+    "pfc_destroy", "pfc_strerror", "pfc_smooth", "pfc_fields",
 )
 
 _lib = None
@@ -58,6 +59,8 @@ def load():
     L.pfc_destroy.argtypes = [vp]
     L.pfc_smooth.argtypes = [dp, dp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                              ctypes.c_double, ctypes.c_int]
+    L.pfc_fields.argtypes = [dp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                             ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, dp, dp]
     L.pfc_strerror.argtypes = [ctypes.c_int]
     L.pfc_strerror.restype = ctypes.c_char_p
     for s in SYMBOLS:
@@ -154,8 +157,8 @@ class Engine:
         return s.value
 
 
-__all__ = ["Engine", "PfcError", "load", "strerror", "smooth", "FP64", "MIXED", "LIB_PATH", "BIN_PATH", "SMOOTH_PATH",
-           "SYMBOLS"]
+__all__ = ["Engine", "PfcError", "load", "strerror", "smooth", "fields", "FP64", "MIXED", "LIB_PATH",
+           "BIN_PATH", "SMOOTH_PATH", "FIELDS_PATH", "SYMBOLS"]
 
 
 def smooth(field, dx, dy, sigma, precision=FP64):
@@ -170,3 +173,21 @@ def smooth(field, dx, dy, sigma, precision=FP64):
     if rc != 0:
         raise PfcError(rc, "pfc_smooth")
     return out
+
+
+def fields(density, dx, dy, fold, sk, sb, power=None):
+    """Orientation field phi (complex, max |phi| = 1) and, when power is
+    given, deformation field chi of one H x W density on the GPU (pfc_fields,
+    the computation of src/fields.c:104-359).  Returns (phi, chi or None)."""
+    L = load()
+    a = np.ascontiguousarray(density, dtype=np.float64)
+    H, W = a.shape
+    phi = np.empty((H, W), dtype=np.complex128)
+    chi = np.empty((H, W)) if power is not None else None
+    dp = ctypes.POINTER(ctypes.c_double)
+    rc = L.pfc_fields(a.ctypes.data_as(dp), W, H, dx, dy, fold, sk, sb,
+                      0.0 if power is None else power, phi.ctypes.data_as(dp),
+                      chi.ctypes.data_as(dp) if chi is not None else None)
+    if rc != 0:
+        raise PfcError(rc, "pfc_fields")
+    return phi, chi
