@@ -66,7 +66,7 @@ typedef struct pfc_params {
 #define PFC_EINVAL (-1)     /* bad argument (size, U, W, pointer) */
 #define PFC_EDEVICE (-2)    /* HIP runtime error */
 #define PFC_ENOMEM (-3)     /* device allocation failed */
-#define PFC_ESIZE (-4)      /* grid size has no kernel (Nx, Ny must be 2^k in [64, 8192]) */
+#define PFC_ESIZE (-4)      /* grid size has no kernel (Nx, Ny: 2^k in [64, 8192] or 384, 768, 1536) */
 #define PFC_ESTATE (-5)     /* call out of order (e.g. step before set_fields) */
 
 typedef struct pfc_ctx pfc_ctx;

@@ -6,9 +6,10 @@
 // transform read/write global memory with unit stride across a wavefront and
 // only the L/P-strided middle stages go through LDS.
 //
-// Stage plan: one leading stage of radix R0 = L / P^s (when L is not a power
-// of P, done in registers with P/R0 butterflies per thread), then s stages of
-// radix P (Stockham autosort).  Twiddles come from a table tw[m] =
+// Stage plan (L = 2^k): one leading stage of radix R0 = L / P^s (when L is
+// not a power of P, done in registers with P/R0 butterflies per thread), then
+// s stages of radix P (Stockham autosort).  L = 3 * 2^k uses P = 12 (or 6)
+// points per thread: a radix-12 (6) stage, then radix-4 and radix-2 stages.  Twiddles come from a table tw[m] =
 // exp(-2 pi i m / (L*TWS)) computed on the host in extended precision; the
 // inverse uses the conjugate.
 #pragma once
@@ -52,46 +53,52 @@ __device__ __forceinline__ cpx<T> cmul_i(cpx<T> a) {
     else return {a.y, -a.x};
 }
 
-// cos/sin(2 pi k / 16), k = 0..15 (folded at compile time)
-__host__ __device__ constexpr double c16(int k) {
-    constexpr double a = 0.92387953251128675613, b = 0.70710678118654752440,
-                     c = 0.38268343236508977173;
-    switch (k & 15) {
-        case 0: return 1.0;  case 1: return a;   case 2: return b;   case 3: return c;
-        case 4: return 0.0;  case 5: return -c;  case 6: return -b;  case 7: return -a;
-        case 8: return -1.0; case 9: return -a;  case 10: return -b; case 11: return -c;
-        case 12: return 0.0; case 13: return c;  case 14: return b;  default: return a;
-    }
+// cos/sin(2 pi k / 96), k = 0..95: every root of unity of an order dividing 96
+// (2, 3, 4, 6, 8, 12, 16, 24, 32), non-recursive so it folds after unrolling
+__host__ __device__ constexpr double c96(int k) {
+    constexpr double t[25] = {1.0,
+                              9.978589232386035067381e-1,
+                              9.914448613738104111446e-1,
+                              9.807852804032304491262e-1,
+                              9.659258262890682867497e-1,
+                              9.469301294951056642558e-1,
+                              9.238795325112867561282e-1,
+                              8.968727415326883038941e-1,
+                              8.660254037844386467637e-1,
+                              8.314696123025452370788e-1,
+                              7.933533402912351645798e-1,
+                              7.518398074789773964075e-1,
+                              7.071067811865475244008e-1,
+                              6.593458151000688684251e-1,
+                              6.087614290087206394161e-1,
+                              5.555702330196022247428e-1,
+                              5.0e-1,
+                              4.422886902190012819952e-1,
+                              3.826834323650897717285e-1,
+                              3.214394653031615807011e-1,
+                              2.588190451025207623489e-1,
+                              1.950903220161282678483e-1,
+                              1.305261922200515915484e-1,
+                              6.540312923014306681532e-2,
+                              0.0};
+    k %= 96;
+    if (k < 0) k += 96;
+    if (k <= 24) return t[k];
+    if (k <= 48) return -t[48 - k];
+    if (k <= 72) return -t[k - 48];
+    return t[96 - k];
 }
-__host__ __device__ constexpr double s16(int k) { return c16(k - 4); }
+__host__ __device__ constexpr double s96(int k) { return c96(k - 24); }
 
-// cos/sin(2 pi k / 32), k = 0..31 (non-recursive so it folds after unrolling)
-__host__ __device__ constexpr double c32(int k) {
-    constexpr double t[9] = {1.0,
-                             0.98078528040323044913,
-                             0.92387953251128675613,
-                             0.83146961230254523708,
-                             0.70710678118654752440,
-                             0.55557023301960222474,
-                             0.38268343236508977173,
-                             0.19509032201612826785,
-                             0.0};
-    k &= 31;
-    if (k <= 8) return t[k];
-    if (k <= 16) return -t[16 - k];
-    if (k <= 24) return -t[k - 16];
-    return t[32 - k];
-}
-__host__ __device__ constexpr double s32(int k) { return c32(k - 8); }
-
-// exp(DIR 2 pi i k / n) for n dividing 32, folded at compile time
+// exp(DIR 2 pi i k / n) for n dividing 96, folded at compile time
 template <typename T>
 __device__ __forceinline__ cpx<T> root(int k, int n, int dir) {
-    const int m = k * (32 / n);
-    return {(T)c32(m), (T)(dir * s32(m))};
+    const int m = k * (96 / n);
+    return {(T)c96(m), (T)(dir * s96(m))};
 }
 
-// In-register DFT of size R (1, 2, 4, 8, 16): v[k] <- sum_r v[r] e^{DIR 2 pi i rk/R}
+// In-register DFT of size R (1, 2, 3, 4, 6, 8, 12, 16):
+// v[k] <- sum_r v[r] e^{DIR 2 pi i rk/R}
 template <int R, int DIR, typename T>
 __device__ __forceinline__ void dft(cpx<T>* v) {
     if constexpr (R == 1) {
@@ -100,6 +107,16 @@ __device__ __forceinline__ void dft(cpx<T>* v) {
         cpx<T> a = v[0], b = v[1];
         v[0] = cadd(a, b);
         v[1] = csub(a, b);
+    } else if constexpr (R == 3) {
+        // X1,2 = v0 - (v1 + v2)/2 +- DIR i sin(2 pi/3) (v1 - v2)
+        const T s3 = (T)8.660254037844386467637e-1;
+        const cpx<T> t1 = cadd(v[1], v[2]);
+        const cpx<T> t2 = csub(v[1], v[2]);
+        const cpx<T> m1 = {v[0].x - (T)0.5 * t1.x, v[0].y - (T)0.5 * t1.y};
+        const cpx<T> m2 = cmul_i<DIR>(cpx<T>{s3 * t2.x, s3 * t2.y});
+        v[0] = cadd(v[0], t1);
+        v[1] = cadd(m1, m2);
+        v[2] = csub(m1, m2);
     } else {
         constexpr int H = R / 2;
         cpx<T> e[H], o[H];
@@ -118,9 +135,7 @@ __device__ __forceinline__ void dft(cpx<T>* v) {
             } else if (4 * k == R) {
                 t = cmul_i<DIR>(o[k]);
             } else {
-                constexpr int step = 16 / R;
-                const cpx<T> w = {(T)c16(k * step), (T)(DIR * s16(k * step))};
-                t = cmul(o[k], w);
+                t = cmul(o[k], root<T>(k, R, DIR));
             }
             v[k] = cadd(e[k], t);
             v[k + H] = csub(e[k], t);
@@ -222,9 +237,33 @@ template <typename T, int L, int TPB, int DIR, int TWS, int LS>
 struct BlockFFT {
     static constexpr int P = L / TPB;
     static_assert(P * TPB == L, "L must be a multiple of TPB");
-    static_assert(P == 2 || P == 4 || P == 8 || P == 16, "P must be 2, 4, 8 or 16");
-    static constexpr int S = ilog(P, L);          // number of radix-P stages
+    static constexpr bool POW2 = (L & (L - 1)) == 0;
+    static_assert(POW2 ? (P == 2 || P == 4 || P == 8 || P == 16) : (P == 6 || P == 12),
+                  "P must be 2, 4, 8 or 16 (L = 2^k) or 6, 12 (L = 3 * 2^k)");
+    static constexpr int S = ilog(P, L);          // number of radix-P stages (L = 2^k)
     static constexpr int R0 = L / ipow(P, S);     // leading stage radix (1 if none)
+
+    // L = 3 * 2^k: stage s has radix gcd(P, L / (radices of stages < s)), so
+    // the first stage takes the factor 3 (radix 12 or 6), the rest radix 4 or 2.
+    static constexpr int gcd_(int a, int b) { return b == 0 ? a : gcd_(b, a % b); }
+    static constexpr int rad(int s) {
+        int rem = L;
+        for (int i = 0; i < s; i++) rem /= gcd_(P, rem);
+        return gcd_(P, rem);
+    }
+    static constexpr int ns(int s) {   // product of the radices of stages < s
+        int n = 1;
+        for (int i = 0; i < s; i++) n *= rad(i);
+        return n;
+    }
+    static constexpr int nstages() {
+        int rem = L, n = 0;
+        while (rem > 1) {
+            rem /= gcd_(P, rem);
+            n++;
+        }
+        return n;
+    }
 
     // Stage twiddle e^{-2 pi i m/L} (conjugated for the inverse): entry m*TWS
     // of a table of L*TWS entries, in global memory or copied to LDS.  Global and LDS reads are
@@ -249,7 +288,8 @@ struct BlockFFT {
 #pragma unroll
         for (int b = 0; b < B; b++) {
             const int j = t + b * TPB;
-            const int k = j & (Ns - 1);
+            // Ns is a power of two for L = 2^k; a compile-time constant in run_mixed
+            const int k = POW2 ? (j & (Ns - 1)) : (j % Ns);
             cpx<T> x[R];
 #pragma unroll
             for (int r = 0; r < R; r++) x[r] = in[b + r * B];
@@ -316,29 +356,49 @@ struct BlockFFT {
     //
     // The twiddle loads take an opaque zero offset, so that the compiler does
     // not share them (and keep them live) across the transforms of a kernel.
+    template <int s, typename TW>
+    __device__ static __forceinline__ void run_mixed(cpx<T> (&v)[P], cpx<T>* lds, TW tw, int t,
+                                                     unsigned z) {
+        constexpr int R = rad(s);
+        constexpr int Ns = ns(s);
+        if constexpr (s == nstages() - 1) {
+            stage<R, true>(v, Ns, lds, tw, t, z);
+        } else {
+            if constexpr (s > 0) __syncthreads();
+            stage<R, false>(v, Ns, lds, tw, t, z);
+            __syncthreads();
+            gather<rad(s + 1)>(v, lds, t);
+            run_mixed<s + 1>(v, lds, tw, t, z);
+        }
+    }
+
     template <typename TW>
     __device__ static __forceinline__ void run(cpx<T> (&v)[P], cpx<T>* lds, TW tw, int t) {
         int Ns = 1;
         unsigned z = 0;
         asm volatile("" : "+s"(z));
         __syncthreads();
-        if constexpr (R0 > 1) {
-            // in-register leading stage: butterfly b of thread t uses v[b + r*(P/R0)]
-            stage<R0, false>(v, 1, lds, tw, t, z);
-            Ns = R0;
-            __syncthreads();
-            gather<P>(v, lds, t);
-        }
-#pragma unroll
-        for (int s = 0; s < S; s++) {
-            if (s == S - 1) {
-                stage<P, true>(v, Ns, lds, tw, t, z);
-            } else {
-                if (s > 0 || R0 > 1) __syncthreads();
-                stage<P, false>(v, Ns, lds, tw, t, z);
-                Ns *= P;
+        if constexpr (!POW2) {
+            run_mixed<0>(v, lds, tw, t, z);
+        } else {
+            if constexpr (R0 > 1) {
+                // in-register leading stage: butterfly b of thread t uses v[b + r*(P/R0)]
+                stage<R0, false>(v, 1, lds, tw, t, z);
+                Ns = R0;
                 __syncthreads();
                 gather<P>(v, lds, t);
+            }
+#pragma unroll
+            for (int s = 0; s < S; s++) {
+                if (s == S - 1) {
+                    stage<P, true>(v, Ns, lds, tw, t, z);
+                } else {
+                    if (s > 0 || R0 > 1) __syncthreads();
+                    stage<P, false>(v, Ns, lds, tw, t, z);
+                    Ns *= P;
+                    __syncthreads();
+                    gather<P>(v, lds, t);
+                }
             }
         }
     }

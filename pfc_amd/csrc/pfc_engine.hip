@@ -12,6 +12,7 @@
 // Everything runs on one HIP stream owned by the context.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <climits>
 #include <cmath>
 #include <cstdio>
@@ -209,7 +210,6 @@ int pfc_create(pfc_ctx** out, const pfc_params* p, int Nx, int Ny, int precision
     c->prec = precision;
     c->p = *p;
     c->nO = 2 * c->U + c->U * (c->U - 1) / 2;
-    c->nk = (size_t)(Nx / 2) * lay_pitch(ilog2(Ny), precision == PFC_FP64 ? 16u : 8u);   // column-major, padded columns
     c->nkn = (size_t)(Nx / 2 + 1) * Ny;   // half spectrum, kx = 0..Nx/2
     c->nr = (size_t)Nx * Ny;
     c->esz = (precision == PFC_FP64) ? 16 : 8;
@@ -218,6 +218,9 @@ int pfc_create(pfc_ctx** out, const pfc_params* p, int Nx, int Ny, int precision
                                         : row_block<float>(Nx, Ny, PFC_I_BLOCKED));
     c->sJ = ilog2(precision == PFC_FP64 ? row_block<double>(Nx, Ny, PFC_J_BLOCKED)
                                         : row_block<float>(Nx, Ny, PFC_J_BLOCKED));
+    // field stride of I, J, K, O: the largest of their layouts (padded strides)
+    c->nk = std::max({lay_elems(c->sC, Nx / 2, Ny, c->esz), lay_elems(c->sI, Nx / 2, Ny, c->esz),
+                      lay_elems(c->sJ, Nx / 2, Ny, c->esz)});
     const size_t rsz = c->esz / 2;
     c->dx = 1.0;
     c->dy = 1.0;

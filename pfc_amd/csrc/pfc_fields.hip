@@ -295,8 +295,11 @@ void row(const Job& j, const C* src, const C* src2, void* dst, const FieldArgs& 
         case 64: row_w<64, MODE>(j, src, src2, dst, fa); break;
         case 128: row_w<128, MODE>(j, src, src2, dst, fa); break;
         case 256: row_w<256, MODE>(j, src, src2, dst, fa); break;
+        case 384: row_w<384, MODE>(j, src, src2, dst, fa); break;
         case 512: row_w<512, MODE>(j, src, src2, dst, fa); break;
+        case 768: row_w<768, MODE>(j, src, src2, dst, fa); break;
         case 1024: row_w<1024, MODE>(j, src, src2, dst, fa); break;
+        case 1536: row_w<1536, MODE>(j, src, src2, dst, fa); break;
         case 2048: row_w<2048, MODE>(j, src, src2, dst, fa); break;
         case 4096: row_w<4096, MODE>(j, src, src2, dst, fa); break;
         case 8192: row_w<8192, MODE>(j, src, src2, dst, fa); break;
@@ -316,8 +319,11 @@ void col(const Job& j, const C* src, C* dst, C* dst2, const FieldArgs& fa) {
         case 64: col_h<64, MODE>(j, src, dst, dst2, fa); break;
         case 128: col_h<128, MODE>(j, src, dst, dst2, fa); break;
         case 256: col_h<256, MODE>(j, src, dst, dst2, fa); break;
+        case 384: col_h<384, MODE>(j, src, dst, dst2, fa); break;
         case 512: col_h<512, MODE>(j, src, dst, dst2, fa); break;
+        case 768: col_h<768, MODE>(j, src, dst, dst2, fa); break;
         case 1024: col_h<1024, MODE>(j, src, dst, dst2, fa); break;
+        case 1536: col_h<1536, MODE>(j, src, dst, dst2, fa); break;
         case 2048: col_h<2048, MODE>(j, src, dst, dst2, fa); break;
         case 4096: col_h<4096, MODE>(j, src, dst, dst2, fa); break;
         case 8192: col_h<8192, MODE>(j, src, dst, dst2, fa); break;

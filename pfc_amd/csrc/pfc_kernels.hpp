@@ -105,36 +105,51 @@ __device__ __forceinline__ double ky_of(int j, int Ny, double dky) {
 }
 
 // Row-spectral layouts.  A buffer of H columns by Ny rows is stored in
-// y-blocks of YB = 2^s rows: element (kx, y) at ((y >> s) H + kx) P + (y mod YB),
-// with pitch P = YB (plus the padding below for long columns).
+// y-blocks of YB = 2^s rows: element (kx, y) at (y >> s) Q + kx P + (y mod YB),
+// with column pitch P and block pitch Q.
 // s = log2(Ny) is the column-major layout kx P + y (whole columns
 // contiguous); s = log2(TR) makes the TR rows of a row-pass block one
 // contiguous run of TR elements per kx.
-// Row-pass lane (kx = t + q TPB, row y): byte offset of q = 0 and the step in q.
-// Columns of 1024 or more rows in the column-major layout are padded by
-// PFC_CPAD bytes.  Unpadded, the row pass's lanes (consecutive kx) sit a
-// power of two apart (64 KiB for 4096 FP64 rows) and crowd a few HBM channels;
-// an odd multiple of 128 bytes spreads them (the row pass at 4096^2 runs
-// 12-14 % faster, the column pass reads whole columns either way).
+// Long strides are padded by PFC_CPAD bytes: P = YB (+ pad when YB >= 1024),
+// Q = H YB (+ pad when YB < 1024).  Unpadded, the lanes of the pass that cuts
+// across the layout (consecutive kx in the row pass on the column-major
+// layout) sit a power of two apart (64 KiB for 4096 FP64 rows) and fall into
+// the same L2 sets and HBM channels, so partial lines are evicted before
+// their other half is read; an odd multiple of 128 bytes spreads them.
 #ifndef PFC_CPAD
 #define PFC_CPAD 384
 #endif
-__host__ __device__ constexpr int ilog2c(int n) { return n > 1 ? 1 + ilog2c(n / 2) : 0; }
+// ceil(log2 n), as the host-side ilog2
+__host__ __device__ constexpr int ilog2c(int n) {
+    int s = 0;
+    while ((1 << s) < n) s++;
+    return s;
+}
 __host__ __device__ constexpr unsigned lay_pitch(int s, unsigned E) {
     return (1u << s) + (s >= 10 ? (unsigned)PFC_CPAD / E : 0u);
 }
+__host__ __device__ constexpr unsigned lay_bpitch(int s, int H, unsigned E) {
+    return (unsigned)H * (1u << s) + (s < 10 ? (unsigned)PFC_CPAD / E : 0u);
+}
+// Row-pass lane (kx = t + q TPB, row y): byte offset of q = 0 and the step in q.
 __device__ __forceinline__ unsigned lay_row_off(int s, int H, int t, int y, unsigned E) {
-    return ((((unsigned)(y >> s) * (unsigned)H + (unsigned)t) * lay_pitch(s, E)) + (unsigned)(y & ((1 << s) - 1))) * E;
+    return ((unsigned)(y >> s) * lay_bpitch(s, H, E) + (unsigned)t * lay_pitch(s, E) +
+            (unsigned)(y & ((1 << s) - 1))) * E;
 }
 __device__ __forceinline__ unsigned lay_row_step(int s, int TPB, unsigned E) {
     return ((unsigned)TPB * lay_pitch(s, E)) * E;
 }
 // Column-pass lane (column kx, y = t + q TPB); needs TPB >= YB or YB >= Ny.
 __device__ __forceinline__ unsigned lay_col_off(int s, int H, int kx, int t, unsigned E) {
-    return ((((unsigned)(t >> s) * (unsigned)H + (unsigned)kx) * lay_pitch(s, E)) + (unsigned)(t & ((1 << s) - 1))) * E;
+    return ((unsigned)(t >> s) * lay_bpitch(s, H, E) + (unsigned)kx * lay_pitch(s, E) +
+            (unsigned)(t & ((1 << s) - 1))) * E;
 }
 __device__ __forceinline__ unsigned lay_col_step(int s, int H, int TPB, unsigned E) {
-    return (TPB >= (1 << s)) ? (unsigned)TPB * (unsigned)H * E : (unsigned)TPB * E;
+    return (TPB >= (1 << s)) ? (unsigned)(TPB >> s) * lay_bpitch(s, H, E) * E : (unsigned)TPB * E;
+}
+// Elements per field of a buffer in layout s (one past the last element).
+inline size_t lay_elems(int s, int H, int Ny, unsigned E) {
+    return (size_t)(((Ny + (1 << s) - 1) >> s) - 1) * lay_bpitch(s, H, E) + (size_t)H * lay_pitch(s, E);
 }
 
 // ------------------------------------------------ elementary functions
@@ -523,7 +538,8 @@ __device__ __forceinline__ void hermitian_cols(cpx<T> (&M)[U][NY / TPB], cpx<T>*
         if (mode != 0) {
 #pragma unroll
             for (int q = 0; q < P; q++) {
-                const cpx<T> pj = L[(NY - (t + q * TPB)) & (NY - 1)];
+                const int jj = t + q * TPB;
+                const cpx<T> pj = L[jj == 0 ? 0 : NY - jj];
                 M[u][q] = cscale(cadd(M[u][q], cconj(pj)), (T)0.5);
             }
         }

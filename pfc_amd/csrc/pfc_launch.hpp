@@ -42,8 +42,11 @@ constexpr RowCfg row_cfg(int nx) {
         case 64: return {8, 32};
         case 128: return {8, 32};
         case 256: return {16, 16};
+        case 384: return {16, 16};   // 3 * 2^k: 12 points per thread
         case 512: return {32, 8};
+        case 768: return {32, 8};
         case 1024: return {64, 4};
+        case 1536: return {64, 4};
         case 2048: return {128, f64 ? 4 : 8};
         case 4096: return {256, f64 ? PFC_ROW_TR4096_F64 : PFC_ROW_TR4096_F32};
         case 8192: return {512, f64 ? 1 : 2};
@@ -58,8 +61,11 @@ constexpr ColCfg col_cfg(int ny) {
         case 64: return {8, 32};
         case 128: return {16, 16};
         case 256: return {32, 8};
+        case 384: return {32, 8};    // 3 * 2^k: 12 points per thread
         case 512: return {64, 4};
+        case 768: return {64, 4};
         case 1024: return {128, 2};
+        case 1536: return {128, 2};
         case 2048: return {256, 1};
         case 4096: return {512, 1};
         case 8192: return {1024, 1};
@@ -157,8 +163,11 @@ struct Launch {
             case 64: row_step_n<64>(c, t, ref); break;
             case 128: row_step_n<128>(c, t, ref); break;
             case 256: row_step_n<256>(c, t, ref); break;
+            case 384: row_step_n<384>(c, t, ref); break;
             case 512: row_step_n<512>(c, t, ref); break;
+            case 768: row_step_n<768>(c, t, ref); break;
             case 1024: row_step_n<1024>(c, t, ref); break;
+            case 1536: row_step_n<1536>(c, t, ref); break;
             case 2048: row_step_n<2048>(c, t, ref); break;
             case 4096: row_step_n<4096>(c, t, ref); break;
             case 8192: row_step_n<8192>(c, t, ref); break;
@@ -195,8 +204,11 @@ struct Launch {
             case 64: col_step_n<64>(c, ref); break;
             case 128: col_step_n<128>(c, ref); break;
             case 256: col_step_n<256>(c, ref); break;
+            case 384: col_step_n<384>(c, ref); break;
             case 512: col_step_n<512>(c, ref); break;
+            case 768: col_step_n<768>(c, ref); break;
             case 1024: col_step_n<1024>(c, ref); break;
+            case 1536: col_step_n<1536>(c, ref); break;
             case 2048: col_step_n<2048>(c, ref); break;
             case 4096: col_step_n<4096>(c, ref); break;
             case 8192: col_step_n<8192>(c, ref); break;
@@ -217,8 +229,11 @@ struct Launch {
             case 64: row_c2r<64>(c, src, sS, dst, K, nbuf); break;
             case 128: row_c2r<128>(c, src, sS, dst, K, nbuf); break;
             case 256: row_c2r<256>(c, src, sS, dst, K, nbuf); break;
+            case 384: row_c2r<384>(c, src, sS, dst, K, nbuf); break;
             case 512: row_c2r<512>(c, src, sS, dst, K, nbuf); break;
+            case 768: row_c2r<768>(c, src, sS, dst, K, nbuf); break;
             case 1024: row_c2r<1024>(c, src, sS, dst, K, nbuf); break;
+            case 1536: row_c2r<1536>(c, src, sS, dst, K, nbuf); break;
             case 2048: row_c2r<2048>(c, src, sS, dst, K, nbuf); break;
             case 4096: row_c2r<4096>(c, src, sS, dst, K, nbuf); break;
             case 8192: row_c2r<8192>(c, src, sS, dst, K, nbuf); break;
@@ -238,8 +253,11 @@ struct Launch {
             case 64: row_r2c<64>(c, src, dst, sD, nbuf); break;
             case 128: row_r2c<128>(c, src, dst, sD, nbuf); break;
             case 256: row_r2c<256>(c, src, dst, sD, nbuf); break;
+            case 384: row_r2c<384>(c, src, dst, sD, nbuf); break;
             case 512: row_r2c<512>(c, src, dst, sD, nbuf); break;
+            case 768: row_r2c<768>(c, src, dst, sD, nbuf); break;
             case 1024: row_r2c<1024>(c, src, dst, sD, nbuf); break;
+            case 1536: row_r2c<1536>(c, src, dst, sD, nbuf); break;
             case 2048: row_r2c<2048>(c, src, dst, sD, nbuf); break;
             case 4096: row_r2c<4096>(c, src, dst, sD, nbuf); break;
             case 8192: row_r2c<8192>(c, src, dst, sD, nbuf); break;
@@ -260,8 +278,11 @@ struct Launch {
             case 64: col_fwd<64>(c, src, sS, dst, nbuf, scale); break;
             case 128: col_fwd<128>(c, src, sS, dst, nbuf, scale); break;
             case 256: col_fwd<256>(c, src, sS, dst, nbuf, scale); break;
+            case 384: col_fwd<384>(c, src, sS, dst, nbuf, scale); break;
             case 512: col_fwd<512>(c, src, sS, dst, nbuf, scale); break;
+            case 768: col_fwd<768>(c, src, sS, dst, nbuf, scale); break;
             case 1024: col_fwd<1024>(c, src, sS, dst, nbuf, scale); break;
+            case 1536: col_fwd<1536>(c, src, sS, dst, nbuf, scale); break;
             case 2048: col_fwd<2048>(c, src, sS, dst, nbuf, scale); break;
             case 4096: col_fwd<4096>(c, src, sS, dst, nbuf, scale); break;
             case 8192: col_fwd<8192>(c, src, sS, dst, nbuf, scale); break;
@@ -282,8 +303,11 @@ struct Launch {
             case 64: col_smooth<64>(c, src, dst, sigma); break;
             case 128: col_smooth<128>(c, src, dst, sigma); break;
             case 256: col_smooth<256>(c, src, dst, sigma); break;
+            case 384: col_smooth<384>(c, src, dst, sigma); break;
             case 512: col_smooth<512>(c, src, dst, sigma); break;
+            case 768: col_smooth<768>(c, src, dst, sigma); break;
             case 1024: col_smooth<1024>(c, src, dst, sigma); break;
+            case 1536: col_smooth<1536>(c, src, dst, sigma); break;
             case 2048: col_smooth<2048>(c, src, dst, sigma); break;
             case 4096: col_smooth<4096>(c, src, dst, sigma); break;
             case 8192: col_smooth<8192>(c, src, dst, sigma); break;
@@ -304,8 +328,11 @@ struct Launch {
             case 64: col_eval<64>(c); break;
             case 128: col_eval<128>(c); break;
             case 256: col_eval<256>(c); break;
+            case 384: col_eval<384>(c); break;
             case 512: col_eval<512>(c); break;
+            case 768: col_eval<768>(c); break;
             case 1024: col_eval<1024>(c); break;
+            case 1536: col_eval<1536>(c); break;
             case 2048: col_eval<2048>(c); break;
             case 4096: col_eval<4096>(c); break;
             case 8192: col_eval<8192>(c); break;

@@ -117,7 +117,8 @@ def rel(a, b):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("W,H", [(64, 64), (128, 64), (64, 256), (512, 512), (1024, 2048)])
+@pytest.mark.parametrize("W,H", [(64, 64), (128, 64), (64, 256), (512, 512), (1024, 2048), (384, 768),
+                                 (1536, 384)])
 @pytest.mark.parametrize("power", [1.0, 2.0])
 def test_matches_oracle(gpu, W, H, power):
     rng = np.random.default_rng(W * 7 + H)

@@ -71,6 +71,22 @@ def test_wide_and_tall_grids(gpu, Nx, Ny, precision):
     assert err < (1e-13 if precision == FP64 else 2e-6), err
 
 
+@pytest.mark.parametrize("precision", [FP64, MIXED])
+@pytest.mark.parametrize("Nx,Ny", [(384, 384), (768, 384), (384, 1536), (1536, 768), (768, 64),
+                                   (64, 1536)])
+def test_three_times_power_of_two_grids(gpu, Nx, Ny, precision):
+    """FFT lengths 3 * 2^k (a radix-12 first stage) in the row and column
+    passes, 3 fields, across the refresh step t = 100, then evaluate."""
+    p = hetero_params()
+    f = hetero_patchwork(Nx, Ny, 0.7, 0.7, domain=48, seed=13)
+    o, g = run_pair(p, f, Nx, Ny, 0.7, 0.7, 98, 4, precision=precision)
+    err = rel_l2(g.get_fields(), o.get_fields())
+    assert err < (1e-13 if precision == FP64 else 2e-6), err
+    fo, no = o.evaluate()
+    fg, ng = g.evaluate()
+    assert abs(fg - fo) <= (1e-12 if precision == FP64 else 1e-6) * abs(fo), (fg, fo)
+
+
 @pytest.mark.parametrize("c", [0.0, 1.0])
 def test_graphene_steps_fp64(gpu, c):
     Nx, Ny = 128, 128

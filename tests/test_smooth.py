@@ -60,7 +60,7 @@ def test_without_gpu_reaches_the_device(tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("Nx,Ny", [(64, 64), (128, 64), (64, 256), (512, 512)])
+@pytest.mark.parametrize("Nx,Ny", [(64, 64), (128, 64), (64, 256), (512, 512), (384, 768)])
 def test_matches_numpy_fp64(gpu, Nx, Ny):
     rng = np.random.default_rng(Nx + Ny)
     f = rng.standard_normal((Ny, Nx))
