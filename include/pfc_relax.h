@@ -149,6 +149,33 @@ int pfc_smooth(const double* in, double* out, int Nx, int Ny, double dx, double 
 int pfc_fields(const double* density, int W, int H, double dx, double dy, int fold, double sk,
                double sb, double power, double* phi, double* chi);
 
+/* ---- batch form: many independent systems stepped together ------------
+ * No counterpart in the reference, which relaxes one system per run
+ * (src/pfc-relax.c:1391-1434); a batch is that run for many systems at once.
+ *
+ * pfc_batch_create groups n existing contexts of one precision (U = 1 or 3;
+ * Nx and Ny each in {256, 384, 512, 768, 1024, 1536, 2048}).  The contexts
+ * keep working through the one-system calls (pfc_set_fields, pfc_get_fields,
+ * pfc_evaluate, pfc_set_box, pfc_nan_step); pfc_batch_step advances all of
+ * them by steps t0 ... t0 + nsteps - 1 exactly as pfc_step would, each pass
+ * one persistent launch per kernel class (FFT length, shape, U) over all its
+ * systems' work and, unless PFC_BATCH_NO_GRAPH,
+ * each step replayed from a captured HIP graph.  PFC_BATCH_AUTOTUNE times both
+ * FFT shapes of every (length, U) and pass present -- points per thread 8 or
+ * 16 (U = 1) and 8 or 4 (U = 3; 8 or 16 at 2048) for 2^k lengths, 12 or 6 (U = 1) and 6 or 12
+ * (U = 3) for 3*2^k -- once per process; without it the first is used unless
+ * this process has tuned the class.  The tuning passes write only the
+ * contexts' scratch buffers.  Destroy the batch before its contexts. */
+#define PFC_BATCH_AUTOTUNE 1
+#define PFC_BATCH_NO_GRAPH 2
+typedef struct pfc_batch pfc_batch;
+int pfc_batch_create(pfc_batch** batch, pfc_ctx* const* ctxs, int n, int flags);
+int pfc_batch_step(pfc_batch* batch, int t0, int nsteps);
+/* points per thread chosen for pass 0 (rows, length Nx) or 1 (columns,
+ * length Ny) of the class (n, U) */
+int pfc_batch_shape(const pfc_batch* batch, int pass, int n, int U, int* points);
+int pfc_batch_destroy(pfc_batch* batch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -4,8 +4,8 @@
     python -m pfc_amd.build --lib      # only the HIP library
 
 Outputs: pfc_amd/lib/libpfcgpu.so and pfc_amd/bin/pfc-relax (rpath to ../lib).
-The HIP translation units (FP64 kernels, FP32 kernels, engine, fields) compile
-in parallel.
+The HIP translation units (FP64 kernels, FP32 kernels, engine, fields, batch)
+compile in parallel.
 """
 from __future__ import annotations
 
@@ -32,7 +32,9 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-pass-failed",
 ]
-HIP_UNITS = ["pfc_launch_f64.hip", "pfc_launch_f32.hip", "pfc_engine.hip", "pfc_fields.hip"]
+HIP_UNITS = ["pfc_launch_f64.hip", "pfc_launch_f32.hip", "pfc_engine.hip", "pfc_fields.hip",
+             "pfc_batch_f32_row.hip", "pfc_batch_f32_col.hip", "pfc_batch_f64_row.hip",
+             "pfc_batch_f64_col.hip", "pfc_batch.hip"]
 HOST_SRCS = ["pfc_relax_main.c", "pfc_io.c"]
 
 

@@ -1,0 +1,201 @@
+// pfc_batch.hpp -- batched relaxation of many independent systems of uneven
+// sizes (pfc_batch_* in include/pfc_relax.h).
+//
+// Systems are grouped into classes (FFT length, points per thread P, fields
+// U).  Each pass of a step launches one persistent kernel per class present:
+// its blocks of kBatchLanes lanes walk the class's work items (a row group, or
+// a column block, of one of its systems) with a grid stride, running the same
+// row and column bodies as the one-system kernels.  P sets the FFT radix (8,
+// 16 or 4 for 2^k; 12 or 6 for 3 * 2^k) and, the block being fixed, the tile
+// (rows or columns per block); the autotuner picks P per (length, U) and pass
+// by timing both candidates.  (One kernel holding every class, dispatching per
+// item, was measured first: register allocation over its 28 inlined bodies
+// spilled 150-690 VGPRs where each body alone fits, and non-inlined bodies
+// save ~150 callee-saved VGPRs to scratch per call.)
+#pragma once
+#include "pfc_launch.hpp"
+
+namespace pfc {
+
+constexpr int kBatchLanes = 256;
+
+// One system as the batch kernels see it (pointers into its context).
+template <typename T>
+struct BatchSys {
+    cpx<T>* I;
+    cpx<T>* J;
+    cpx<T>* K;
+    cpx<T>* Nh;
+    size_t fstride, nstride;
+    int H, sI, sJ, pad;
+    const cpx<T>* twx;
+    const cpx<T>* twy;
+    const T* ktab;
+    const DevParams* prm;
+    int* nan;
+};
+
+// a row group (row pass) or column block (column pass) of one system
+struct BatchItem {
+    int sys, blk;
+};
+
+// Kernel classes (id, length n, points per thread p, fields U).  Each (n, U)
+// has two candidate shapes for the autotuner; three-field classes mostly try
+// fewer points per thread (their registers hold three fields), except at
+// 2048, where 4 points per thread would need more than kBatchLanes lanes
+// per column.
+#define PFC_BATCH_CLASSES(X)                                                                       \
+    X(0, 256, 8, 1) X(1, 256, 16, 1) X(2, 256, 8, 3) X(3, 256, 4, 3)                               \
+    X(4, 384, 12, 1) X(5, 384, 6, 1) X(6, 384, 6, 3) X(7, 384, 12, 3)                              \
+    X(8, 512, 8, 1) X(9, 512, 16, 1) X(10, 512, 8, 3) X(11, 512, 4, 3)                             \
+    X(12, 768, 12, 1) X(13, 768, 6, 1) X(14, 768, 6, 3) X(15, 768, 12, 3)                          \
+    X(16, 1024, 8, 1) X(17, 1024, 16, 1) X(18, 1024, 8, 3) X(19, 1024, 4, 3)                       \
+    X(20, 1536, 12, 1) X(21, 1536, 6, 1) X(22, 1536, 6, 3) X(23, 1536, 12, 3)                      \
+    X(24, 2048, 8, 1) X(25, 2048, 16, 1) X(26, 2048, 8, 3) X(27, 2048, 16, 3)
+constexpr int kBatchClasses = 28;
+struct BatchClass {
+    int n, p, u;
+};
+#define PFC_CLASS_ENTRY(ID, N, PP, U) {N, PP, U},
+constexpr BatchClass kBatchClass[kBatchClasses] = {PFC_BATCH_CLASSES(PFC_CLASS_ENTRY)};
+#undef PFC_CLASS_ENTRY
+
+// rows per row item and columns per column item of a class
+constexpr int batch_tr(int c) { return kBatchLanes / (kBatchClass[c].n / 2 / kBatchClass[c].p); }
+constexpr int batch_tc(int c) { return kBatchLanes / (kBatchClass[c].n / kBatchClass[c].p); }
+
+#ifndef PFC_BATCH_WAVES
+#define PFC_BATCH_WAVES 2
+#endif
+
+// Persistent row pass of one class: the blocks walk the class's items with a
+// grid stride.
+template <typename T, int N, int PP, int U, bool REF>
+__global__ __launch_bounds__(kBatchLanes) PFC_WAVES(PFC_BATCH_WAVES) void k_batch_row(
+    const BatchItem* __restrict__ items, int nitems, const BatchSys<T>* __restrict__ sys,
+    const int* __restrict__ tdev) {
+    constexpr int TPB = N / 2 / PP, TR = kBatchLanes / TPB;
+    __shared__ cpx<T> lds[N / 2 * TR];
+    const int tstep = *tdev;
+    for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
+        const BatchItem it = items[w];
+        const BatchSys<T>& S = sys[it.sys];
+        row_step_body<T, N, TPB, TR, U, REF>(lds, it.blk, S.I, S.J, S.K, S.fstride, S.sI, S.sJ,
+                                             S.twx, S.prm, S.nan, tstep);
+    }
+}
+
+// Persistent column pass of one class.
+template <typename T, int N, int PP, int U, bool REF>
+__global__ __launch_bounds__(kBatchLanes) PFC_WAVES(PFC_BATCH_WAVES) void k_batch_col(
+    const BatchItem* __restrict__ items, int nitems, const BatchSys<T>* __restrict__ sys) {
+    constexpr int TPB = N / PP, TC = kBatchLanes / TPB;
+    constexpr bool LTAB = col_ltab<T, N, TC>();
+    __shared__ cpx<T> lds[N * TC];
+    __shared__ __attribute__((aligned(16))) cpx<T> ltw[LTAB ? N : 1];
+    __shared__ __attribute__((aligned(16))) T lgy[LTAB ? N : 1];
+    for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
+        const BatchItem it = items[w];
+        const BatchSys<T>& S = sys[it.sys];
+        __syncthreads();   // the previous item's last reads of the tables
+        col_step_body<T, N, TPB, TC, U, REF, LTAB>(lds, ltw, lgy, it.blk, S.J, S.K, S.Nh, S.I,
+                                                   S.fstride, S.nstride, S.H, S.sI, S.sJ, S.twy,
+                                                   S.ktab, S.prm);
+    }
+}
+
+// advances the device step counter read by the row pass (one per step)
+__global__ void k_batch_tick(int* tdev);
+
+template <typename T>
+struct BatchLaunch {
+    static void row(int cls, bool ref, const BatchItem* items, int nitems, const BatchSys<T>* sys,
+                    const int* tdev, int grid, hipStream_t s);
+    static void col(int cls, bool ref, const BatchItem* items, int nitems, const BatchSys<T>* sys,
+                    int grid, hipStream_t s);
+    // resident blocks per CU of a class's row or column kernel
+    static int row_occupancy(int cls);
+    static int col_occupancy(int cls);
+    static int occupancy(int cls, bool row_pass) { return row_pass ? row_occupancy(cls) : col_occupancy(cls); }
+};
+
+// Out-of-line definitions, compiled only where BatchLaunch's members are
+// instantiated (pfc_batch_{f32,f64}_{row,col}.hip).
+#ifdef PFC_BATCH_INSTANTIATE
+template <typename T>
+void BatchLaunch<T>::row(int cls, bool ref, const BatchItem* items, int nitems,
+                         const BatchSys<T>* sys, const int* tdev, int grid, hipStream_t s) {
+    switch (cls) {
+#define PFC_LROW(ID, N, PP, U)                                                                   \
+    case ID:                                                                                     \
+        if (ref)                                                                                 \
+            hipLaunchKernelGGL((k_batch_row<T, N, PP, U, true>), dim3(grid), dim3(kBatchLanes), 0, \
+                               s, items, nitems, sys, tdev);                                     \
+        else                                                                                     \
+            hipLaunchKernelGGL((k_batch_row<T, N, PP, U, false>), dim3(grid), dim3(kBatchLanes), \
+                               0, s, items, nitems, sys, tdev);                                  \
+        break;
+        PFC_BATCH_CLASSES(PFC_LROW)
+#undef PFC_LROW
+    }
+}
+template <typename T>
+void BatchLaunch<T>::col(int cls, bool ref, const BatchItem* items, int nitems,
+                         const BatchSys<T>* sys, int grid, hipStream_t s) {
+    switch (cls) {
+#define PFC_LCOL(ID, N, PP, U)                                                                   \
+    case ID:                                                                                     \
+        if (ref)                                                                                 \
+            hipLaunchKernelGGL((k_batch_col<T, N, PP, U, true>), dim3(grid), dim3(kBatchLanes), 0, \
+                               s, items, nitems, sys);                                           \
+        else                                                                                     \
+            hipLaunchKernelGGL((k_batch_col<T, N, PP, U, false>), dim3(grid), dim3(kBatchLanes), \
+                               0, s, items, nitems, sys);                                        \
+        break;
+        PFC_BATCH_CLASSES(PFC_LCOL)
+#undef PFC_LCOL
+    }
+}
+inline int lower_positive(int a, int b, hipError_t e1, hipError_t e2) {
+    if (e1 != hipSuccess || e2 != hipSuccess) return 1;
+    const int m = a < b ? a : b;
+    return m > 0 ? m : 1;
+}
+template <typename T>
+int BatchLaunch<T>::row_occupancy(int cls) {
+    int a = 0, b = 0;
+    hipError_t e1 = hipErrorInvalidValue, e2 = hipErrorInvalidValue;
+    switch (cls) {
+#define PFC_LOCC(ID, N, PP, U)                                                                     \
+    case ID:                                                                                       \
+        e1 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_batch_row<T, N, PP, U, false>,     \
+                                                          kBatchLanes, 0);                         \
+        e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_batch_row<T, N, PP, U, true>,      \
+                                                          kBatchLanes, 0);                         \
+        break;
+        PFC_BATCH_CLASSES(PFC_LOCC)
+#undef PFC_LOCC
+    }
+    return lower_positive(a, b, e1, e2);
+}
+template <typename T>
+int BatchLaunch<T>::col_occupancy(int cls) {
+    int a = 0, b = 0;
+    hipError_t e1 = hipErrorInvalidValue, e2 = hipErrorInvalidValue;
+    switch (cls) {
+#define PFC_LOCC(ID, N, PP, U)                                                                     \
+    case ID:                                                                                       \
+        e1 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_batch_col<T, N, PP, U, false>,     \
+                                                          kBatchLanes, 0);                         \
+        e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_batch_col<T, N, PP, U, true>,      \
+                                                          kBatchLanes, 0);                         \
+        break;
+        PFC_BATCH_CLASSES(PFC_LOCC)
+#undef PFC_LOCC
+    }
+    return lower_positive(a, b, e1, e2);
+}
+#endif
+
+}  // namespace pfc

@@ -411,21 +411,23 @@ __device__ __forceinline__ void nonlinear(const DevParams* __restrict__ p, T (&m
 // row index fastest, so the TR rows' elements of one kx are adjacent lanes
 // and adjacent addresses in the column-major buffers (TR*sizeof(E) bytes
 // contiguous per kx).
+// Body of the row pass for row group g (rows g*TR ... g*TR + TR - 1) on a
+// block of TPB*TR lanes; lds holds H*TR elements.  Shared by the one-system
+// kernel below and the batch kernels.
 template <typename T, int NX, int TPB, int TR, int U, bool REFRESH>
-__global__ __launch_bounds__(TPB* TR) PFC_WAVES(row_waves<T>()) void k_row_step(
-    const cpx<T>* __restrict__ I, cpx<T>* __restrict__ J, cpx<T>* __restrict__ K,
-    size_t fstride, int sI, int sJ, const cpx<T>* __restrict__ twx_,
+__device__ __forceinline__ void row_step_body(
+    cpx<T>* lds, int g, const cpx<T>* __restrict__ I, cpx<T>* __restrict__ J,
+    cpx<T>* __restrict__ K, size_t fstride, int sI, int sJ, const cpx<T>* __restrict__ twx_,
     const DevParams* __restrict__ prm, int* __restrict__ nan_step, int tstep) {
     constexpr int H = NX / 2;
     constexpr int P = H / TPB;
     constexpr unsigned E = sizeof(cpx<T>);
     using FFTF = BlockFFT<T, H, TPB, -1, 2, TR>;
     using FFTI = BlockFFT<T, H, TPB, +1, 2, TR>;
-    __shared__ cpx<T> lds[H * TR];
     const int tid = threadIdx.x;
     const int r = tid % TR;
     const int t = tid / TR;
-    const int y = xcd_remap(blockIdx.x, gridDim.x) * TR + r;
+    const int y = g * TR + r;
     cpx<T>* L = lds + r;
     const GBuf twx = gbuf(twx_);
     const unsigned ioff = lay_row_off(sI, H, t, y, E), istep = lay_row_step(sI, TPB, E);
@@ -488,6 +490,16 @@ __global__ __launch_bounds__(TPB* TR) PFC_WAVES(row_waves<T>()) void k_row_step(
         for (int q = 0; q < P; q++) gstore<T>(dst, voff, q * qstep, m[u][q]);
     }
     PFC_PROBE_AT(0, 4);
+}
+
+template <typename T, int NX, int TPB, int TR, int U, bool REFRESH>
+__global__ __launch_bounds__(TPB* TR) PFC_WAVES(row_waves<T>()) void k_row_step(
+    const cpx<T>* __restrict__ I, cpx<T>* __restrict__ J, cpx<T>* __restrict__ K,
+    size_t fstride, int sI, int sJ, const cpx<T>* __restrict__ twx_,
+    const DevParams* __restrict__ prm, int* __restrict__ nan_step, int tstep) {
+    __shared__ cpx<T> lds[NX / 2 * TR];
+    row_step_body<T, NX, TPB, TR, U, REFRESH>(lds, xcd_remap(blockIdx.x, gridDim.x), I, J, K,
+                                              fstride, sI, sJ, twx_, prm, nan_step, tstep);
 }
 
 // Column addressing modes of the column passes.  mode 0: an ordinary column
@@ -557,22 +569,20 @@ __device__ __forceinline__ void hermitian_cols(cpx<T> (&M)[U][NY / TPB], cpx<T>*
 #ifndef PFC_COL_QB_F32
 #define PFC_COL_QB_F32 2
 #endif
-template <typename T, int NY, int TPB, int TC, int U, bool REFRESH>
-__global__ __launch_bounds__(TPB* TC) PFC_WAVES(col_waves<T>()) void k_col_step(
-    const cpx<T>* __restrict__ J, const cpx<T>* __restrict__ K, cpx<T>* __restrict__ Nh,
-    cpx<T>* __restrict__ I, size_t fstride, size_t nstride, int H, int sI, int sJ,
-    const cpx<T>* __restrict__ twy_, const T* __restrict__ ktab_,
-    const DevParams* __restrict__ prm) {
+// Body of the column pass for column block cb (H/TC + 1 per system) on a
+// block of TPB*TC lanes.  lds holds NY*TC elements; with LTAB, ltw (NY
+// elements, 16-byte aligned) and lgy (NY reals) hold the twiddle and ky-factor
+// tables.  Shared by the one-system kernel below and the batch kernels.
+template <typename T, int NY, int TPB, int TC, int U, bool REFRESH, bool LTAB>
+__device__ __forceinline__ void col_step_body(
+    cpx<T>* lds, cpx<T>* ltw, T* lgy, int cb, const cpx<T>* __restrict__ J,
+    const cpx<T>* __restrict__ K, cpx<T>* __restrict__ Nh, cpx<T>* __restrict__ I, size_t fstride,
+    size_t nstride, int H, int sI, int sJ, const cpx<T>* __restrict__ twy_,
+    const T* __restrict__ ktab_, const DevParams* __restrict__ prm) {
     constexpr int P = NY / TPB;
     constexpr unsigned E = sizeof(cpx<T>);
     using FFTF = BlockFFT<T, NY, TPB, -1, 1, 1>;
     using FFTI = BlockFFT<T, NY, TPB, +1, 1, 1>;
-    // twiddle and ky-factor tables in LDS where they fit beside the
-    // exchange buffer (all sizes but FP64 Ny = 8192)
-    constexpr bool LTAB = NY * TC * E + NY * E + NY * sizeof(T) <= 160 * 1024;
-    __shared__ cpx<T> lds[NY * TC];
-    __shared__ __attribute__((aligned(16))) cpx<T> ltw[LTAB ? NY : 1];   // e^{-2 pi i m/Ny}
-    __shared__ __attribute__((aligned(16))) T lgy[LTAB ? NY : 1];        // exp(-ky^2/sigma^2)
     const int tid = threadIdx.x;
     const int c = tid / TPB;
     const int t = tid % TPB;
@@ -581,7 +591,7 @@ __global__ __launch_bounds__(TPB* TC) PFC_WAVES(col_waves<T>()) void k_col_step(
     const GBuf ktab = gbuf(ktab_);
     PFC_PROBE_AT(65536, 0);
     {
-        const ColJob jb = col_job(blockIdx.x, c, TC, H);
+        const ColJob jb = col_job(cb, c, TC, H);
         const int kx = jb.kx;
         // byte offsets of this lane's first element in a field's buffers
         const unsigned vr = lay_col_off(sJ, H, jb.mode ? 0 : kx, t, E);
@@ -646,7 +656,9 @@ __global__ __launch_bounds__(TPB* TC) PFC_WAVES(col_waves<T>()) void k_col_step(
             // State loads in batches of QB k points, each batch's loads issued
             // together (a load issued after a store completes after it).
             constexpr int QB0 = sizeof(T) == 8 ? PFC_COL_QB_F64 : PFC_COL_QB_F32;
-            constexpr int QB = (QB0 < P) ? QB0 : P;
+            // the largest of QB0, 2, 1 that divides P (P = 6 takes 2)
+            constexpr int QB = (P % QB0 == 0) ? QB0 : (P % 2 == 0 ? 2 : 1);
+            static_assert(P % QB == 0, "state batches must tile the lane's points");
 #pragma unroll
             for (int q0 = 0; q0 < P; q0 += QB) {
                 cpx<T> Nb[QB][U];
@@ -681,7 +693,7 @@ __global__ __launch_bounds__(TPB* TC) PFC_WAVES(col_waves<T>()) void k_col_step(
         PFC_PROBE_AT(65536, 3);
         if constexpr (sizeof(T) == 4) {
             // kx = 0 and kx = H: replace the stored state by its Hermitian part
-            if (blockIdx.x == 0 || blockIdx.x == H / TC) {
+            if (cb == 0 || cb == H / TC) {
                 hermitian_cols<T, NY, TPB, U>(M, L, t, jb.live ? jb.mode : 0);
                 if (jb.live && jb.mode != 0) {
 #pragma unroll
@@ -704,6 +716,27 @@ __global__ __launch_bounds__(TPB* TC) PFC_WAVES(col_waves<T>()) void k_col_step(
         }
     }
     PFC_PROBE_AT(65536, 4);
+}
+
+// twiddle and ky-factor tables in LDS where they fit beside the exchange
+// buffer (all sizes but FP64 Ny = 8192)
+template <typename T, int NY, int TC>
+constexpr bool col_ltab() {
+    return NY * TC * sizeof(cpx<T>) + NY * sizeof(cpx<T>) + NY * sizeof(T) <= 160 * 1024;
+}
+
+template <typename T, int NY, int TPB, int TC, int U, bool REFRESH>
+__global__ __launch_bounds__(TPB* TC) PFC_WAVES(col_waves<T>()) void k_col_step(
+    const cpx<T>* __restrict__ J, const cpx<T>* __restrict__ K, cpx<T>* __restrict__ Nh,
+    cpx<T>* __restrict__ I, size_t fstride, size_t nstride, int H, int sI, int sJ,
+    const cpx<T>* __restrict__ twy_, const T* __restrict__ ktab_,
+    const DevParams* __restrict__ prm) {
+    constexpr bool LTAB = col_ltab<T, NY, TC>();
+    __shared__ cpx<T> lds[NY * TC];
+    __shared__ __attribute__((aligned(16))) cpx<T> ltw[LTAB ? NY : 1];   // e^{-2 pi i m/Ny}
+    __shared__ __attribute__((aligned(16))) T lgy[LTAB ? NY : 1];        // exp(-ky^2/sigma^2)
+    col_step_body<T, NY, TPB, TC, U, REFRESH, LTAB>(lds, ltw, lgy, blockIdx.x, J, K, Nh, I, fstride,
+                                                    nstride, H, sI, sJ, twy_, ktab_, prm);
 }
 
 // ------------------------------------------------ single-buffer row passes

@@ -24,7 +24,11 @@ SYMBOLS = (
     "pfc_create", "pfc_set_fields", "pfc_set_box", "pfc_step", "pfc_evaluate",
     "pfc_get_fields", "pfc_get_energy", "pfc_nan_step", "pfc_timing", "pfc_timing_read",
     "pfc_destroy", "pfc_strerror", "pfc_smooth", "pfc_fields",
+    "pfc_batch_create", "pfc_batch_step", "pfc_batch_shape", "pfc_batch_destroy",
 )
+
+BATCH_AUTOTUNE = 1
+BATCH_NO_GRAPH = 2
 
 _lib = None
 
@@ -61,6 +65,11 @@ def load():
                              ctypes.c_double, ctypes.c_int]
     L.pfc_fields.argtypes = [dp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                              ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, dp, dp]
+    L.pfc_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int]
+    L.pfc_batch_step.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.pfc_batch_shape.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                  ctypes.POINTER(ctypes.c_int)]
+    L.pfc_batch_destroy.argtypes = [vp]
     L.pfc_strerror.argtypes = [ctypes.c_int]
     L.pfc_strerror.restype = ctypes.c_char_p
     for s in SYMBOLS:
@@ -157,8 +166,9 @@ class Engine:
         return s.value
 
 
-__all__ = ["Engine", "PfcError", "load", "strerror", "smooth", "fields", "FP64", "MIXED", "LIB_PATH",
-           "BIN_PATH", "SMOOTH_PATH", "FIELDS_PATH", "SYMBOLS"]
+__all__ = ["Engine", "Batch", "PfcError", "load", "strerror", "smooth", "fields", "FP64", "MIXED",
+           "LIB_PATH", "BIN_PATH", "SMOOTH_PATH", "FIELDS_PATH", "SYMBOLS", "BATCH_AUTOTUNE",
+           "BATCH_NO_GRAPH"]
 
 
 def smooth(field, dx, dy, sigma, precision=FP64):
@@ -191,3 +201,48 @@ def fields(density, dx, dy, fold, sk, sb, power=None):
     if rc != 0:
         raise PfcError(rc, "pfc_fields")
     return phi, chi
+
+
+class Batch:
+    """Many Engines (same precision, U = 1 or 3, sides in {256, 384, 512, 768,
+    1024, 1536, 2048}) stepped together by pfc_batch_step; the engines stay
+    usable on their own (fields, evaluate) and must outlive the batch."""
+
+    def __init__(self, engines, flags=0):
+        L = load()
+        self.engines = list(engines)
+        arr = (ctypes.c_void_p * len(self.engines))(*[e._h.value for e in self.engines])
+        self._h = ctypes.c_void_p()
+        rc = L.pfc_batch_create(ctypes.byref(self._h), arr, len(self.engines), flags)
+        if rc != 0:
+            raise PfcError(rc, "pfc_batch_create")
+
+    def step(self, t0, nsteps):
+        rc = load().pfc_batch_step(self._h, t0, nsteps)
+        if rc != 0:
+            raise PfcError(rc, "pfc_batch_step")
+
+    def shape(self, pass_, n, U):
+        """points per thread chosen for pass 0 (rows) or 1 (columns) of (n, U)"""
+        p = ctypes.c_int()
+        rc = load().pfc_batch_shape(self._h, pass_, n, U, ctypes.byref(p))
+        if rc != 0:
+            raise PfcError(rc, "pfc_batch_shape")
+        return p.value
+
+    def close(self):
+        if self._h:
+            load().pfc_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
