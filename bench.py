@@ -14,7 +14,9 @@ Prints one JSON line.  `roofline` is for the dominant kernel (the longer of
 the row and column passes of the mixed step), timed with HIP events on the
 engine's stream over the timed region; `mixed_vs_fp64_run` compares the two
 forms from the same start over the workload's 5000 steps; `cpu_baseline`
-times the CPU oracle on a bounded sample of the same workload.
+times the CPU oracle on a bounded sample of the same workload; `batch` is the
+batch form on 96 systems of uneven sizes (2^k and 3*2^k sides from 256 to
+2048, one or three fields), beside the same systems stepped one by one.
 """
 from __future__ import annotations
 
@@ -156,6 +158,78 @@ def cpu_baseline(params, fields, Nx, Ny, dx, dy, budget_s):
                       f"(oracle/pfc_oracle.c, OpenMP, {threads} threads), {el:.1f} s"}
 
 
+BATCH_SIDES = (256, 384, 512, 768, 1024, 1536, 2048)
+
+
+def batch_bench(nsys, steps, warmup, seed=96):
+    """The batch form on the batched-relaxation workload: nsys square systems,
+    sides drawn from BATCH_SIDES (2^k and 3*2^k FFT lengths), 70 % one-field
+    graphene / 30 % three-field graphene/hBN patchworks, random lattice
+    orientations and noise, mixed precision, autotuned; beside it the same
+    systems stepped one by one through pfc_step."""
+    from pfc_amd import MIXED, LATTICE_A, graphene_params, hetero_params
+    from pfc_amd.engine import BATCH_AUTOTUNE, Batch, Engine
+    from pfc_amd.inputs import crystal, hetero_patchwork
+    rng = np.random.default_rng(seed)
+    specs = [(int(rng.choice(BATCH_SIDES)), 3 if rng.random() < 0.3 else 1) for _ in range(nsys)]
+    t0 = time.perf_counter()
+    engines = []
+    for k, (n, U) in enumerate(specs):
+        if U == 1:
+            p = graphene_params()
+            dx = LATTICE_A / 8.0
+            f = crystal(n, n, dx, dx, 0.2, -0.2, LATTICE_A, sym=3,
+                        theta=float(rng.uniform(0.0, np.pi / 3)), noise=0.01, seed=k)[None]
+        else:
+            p = hetero_params(dt=0.25, c=1.0, sigma=2.0 / 7.2552)
+            dx = 0.7
+            f = hetero_patchwork(n, n, dx, dx, domain=128, seed=k, noise=0.01)
+        e = Engine(p, n, n, dx, dx, MIXED)
+        e.set_fields(f)
+        engines.append(e)
+    gen_s = time.perf_counter() - t0
+    gp = sum(n * n for n, _ in specs)
+    t0 = time.perf_counter()
+    b = Batch(engines, BATCH_AUTOTUNE)
+    tune_s = time.perf_counter() - t0
+    b.step(0, warmup)
+    t0 = time.perf_counter()
+    b.step(warmup, steps)
+    wall = time.perf_counter() - t0
+    shapes = {f"{n}x{n} U={U}": {"row_points": b.shape(0, n, U), "column_points": b.shape(1, n, U)}
+              for n, U in sorted(set(specs))}
+    b.close()
+    t = warmup + steps
+    for e in engines:
+        e.step(t, 2)
+    for e in engines:
+        e.nan_step()
+    t += 2
+    t0 = time.perf_counter()
+    for e in engines:
+        e.step(t, steps)
+    for e in engines:
+        e.nan_step()
+    wall1 = time.perf_counter() - t0
+    for e in engines:
+        e.close()
+    return {
+        "workload": f"{nsys} systems, sides drawn from {list(BATCH_SIDES)} (seed {seed}), "
+                    f"{sum(1 for _, u in specs if u == 1)} one-field / "
+                    f"{sum(1 for _, u in specs if u == 3)} three-field, mixed precision",
+        "grid_points": gp,
+        "value": gp * steps / wall,
+        "unit": "grid-point-steps/s",
+        "ms_per_step": 1e3 * wall / steps,
+        "steps": steps,
+        "one_by_one": {"value": gp * steps / wall1, "ms_per_step": 1e3 * wall1 / steps},
+        "speedup_vs_one_by_one": wall1 / wall,
+        "autotune_s": tune_s,
+        "shapes": shapes,
+        "input_generation_s": gen_s,
+    }
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -166,6 +240,9 @@ def main(argv=None):
                     help="seconds of CPU-oracle stepping for the baseline (0: skip)")
     ap.add_argument("--accuracy-steps", type=int, default=5000,
                     help="length of the mixed-vs-FP64 comparison run (0: skip)")
+    ap.add_argument("--batch-systems", type=int, default=96,
+                    help="systems of the batched-relaxation leg (0: skip)")
+    ap.add_argument("--batch-steps", type=int, default=100)
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported "
@@ -212,6 +289,10 @@ def main(argv=None):
         acc = accuracy_study(params, fields, Nx, Ny, dx, dy, a.accuracy_steps,
                              (100, 500, 1000, 2000, 5000, 10000, 20000))
 
+    batch = None
+    if a.batch_systems > 0:
+        batch = batch_bench(a.batch_systems, a.batch_steps, 10)
+
     cpu = None
     if a.cpu_budget > 0:
         cpu = cpu_baseline(params, fields, Nx, Ny, dx, dy, a.cpu_budget)
@@ -257,6 +338,7 @@ def main(argv=None):
                      "kernel": ("k_col_step (column pass, mixed)" if dom == "column"
                                 else "k_row_step (row pass, mixed)")},
         "cpu_baseline": cpu,
+        "batch": batch,
         "input_generation_s": gen_s,
     }
     print(json.dumps(out))

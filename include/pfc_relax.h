@@ -157,14 +157,17 @@ int pfc_fields(const double* density, int W, int H, double dx, double dy, int fo
  * Nx and Ny each in {256, 384, 512, 768, 1024, 1536, 2048}).  The contexts
  * keep working through the one-system calls (pfc_set_fields, pfc_get_fields,
  * pfc_evaluate, pfc_set_box, pfc_nan_step); pfc_batch_step advances all of
- * them by steps t0 ... t0 + nsteps - 1 exactly as pfc_step would, each pass
- * one persistent launch per kernel class (FFT length, shape, U) over all its
- * systems' work and, unless PFC_BATCH_NO_GRAPH,
- * each step replayed from a captured HIP graph.  PFC_BATCH_AUTOTUNE times both
+ * them by steps t0 ... t0 + nsteps - 1 exactly as pfc_step would.  Systems of
+ * equal (Nx, Ny, U) are cut into chunks of a cache-sized working set; each
+ * chunk's row and column passes are persistent launches over all its
+ * systems' work, chained on one of four streams, and unless
+ * PFC_BATCH_NO_GRAPH, blocks of 20 steps (single steps where t0 or nsteps do
+ * not align) are replayed from captured HIP graphs.  PFC_BATCH_AUTOTUNE times both
  * FFT shapes of every (length, U) and pass present -- points per thread 8 or
- * 16 (U = 1) and 8 or 4 (U = 3; 8 or 16 at 2048) for 2^k lengths, 12 or 6 (U = 1) and 6 or 12
- * (U = 3) for 3*2^k -- once per process; without it the first is used unless
- * this process has tuned the class.  The tuning passes write only the
+ * 16 (U = 1) and 8 or 4 (U = 3) for 2^k lengths, 12 or 6 (U = 1) and 6 or 12
+ * (U = 3) for 3*2^k -- once per process, and times chunk budgets of 8 to 64
+ * MiB for the batch; without it the first shape is used unless this process
+ * has tuned the class, and 16 MiB chunks.  The tuning passes write only the
  * contexts' scratch buffers.  Destroy the batch before its contexts. */
 #define PFC_BATCH_AUTOTUNE 1
 #define PFC_BATCH_NO_GRAPH 2

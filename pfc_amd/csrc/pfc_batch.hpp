@@ -2,13 +2,13 @@
 // sizes (pfc_batch_* in include/pfc_relax.h).
 //
 // Systems are grouped into classes (FFT length, points per thread P, fields
-// U).  Each pass of a step launches one persistent kernel per class present:
-// its blocks of kBatchLanes lanes walk the class's work items (a row group, or
-// a column block, of one of its systems) with a grid stride, running the same
-// row and column bodies as the one-system kernels.  P sets the FFT radix (8,
-// 16 or 4 for 2^k; 12 or 6 for 3 * 2^k) and, the block being fixed, the tile
-// (rows or columns per block); the autotuner picks P per (length, U) and pass
-// by timing both candidates.  (One kernel holding every class, dispatching per
+// U).  Each pass of a step launches one persistent kernel per group of
+// systems of one class: its blocks walk the group's work items (a row group,
+// or a column block, of one of its systems) with a grid stride, running the
+// same row and column bodies and tiles as the one-system kernels.  P sets the
+// FFT radix (8, 16 or 4 for 2^k; 12 or 6 for 3 * 2^k) and with it the lanes
+// per transform; the autotuner picks P per (length, U) and pass by timing
+// both candidates.  (One kernel holding every class, dispatching per
 // item, was measured first: register allocation over its 28 inlined bodies
 // spilled 150-690 VGPRs where each body alone fits, and non-inlined bodies
 // save ~150 callee-saved VGPRs to scratch per call.)
@@ -16,8 +16,6 @@
 #include "pfc_launch.hpp"
 
 namespace pfc {
-
-constexpr int kBatchLanes = 256;
 
 // One system as the batch kernels see it (pointers into its context).
 template <typename T>
@@ -41,10 +39,8 @@ struct BatchItem {
 };
 
 // Kernel classes (id, length n, points per thread p, fields U).  Each (n, U)
-// has two candidate shapes for the autotuner; three-field classes mostly try
-// fewer points per thread (their registers hold three fields), except at
-// 2048, where 4 points per thread would need more than kBatchLanes lanes
-// per column.
+// has two candidate shapes for the autotuner; three-field classes try fewer
+// points per thread (their registers hold three fields).
 #define PFC_BATCH_CLASSES(X)                                                                       \
     X(0, 256, 8, 1) X(1, 256, 16, 1) X(2, 256, 8, 3) X(3, 256, 4, 3)                               \
     X(4, 384, 12, 1) X(5, 384, 6, 1) X(6, 384, 6, 3) X(7, 384, 12, 3)                              \
@@ -52,7 +48,7 @@ struct BatchItem {
     X(12, 768, 12, 1) X(13, 768, 6, 1) X(14, 768, 6, 3) X(15, 768, 12, 3)                          \
     X(16, 1024, 8, 1) X(17, 1024, 16, 1) X(18, 1024, 8, 3) X(19, 1024, 4, 3)                       \
     X(20, 1536, 12, 1) X(21, 1536, 6, 1) X(22, 1536, 6, 3) X(23, 1536, 12, 3)                      \
-    X(24, 2048, 8, 1) X(25, 2048, 16, 1) X(26, 2048, 8, 3) X(27, 2048, 16, 3)
+    X(24, 2048, 8, 1) X(25, 2048, 16, 1) X(26, 2048, 8, 3) X(27, 2048, 4, 3)
 constexpr int kBatchClasses = 28;
 struct BatchClass {
     int n, p, u;
@@ -61,23 +57,33 @@ struct BatchClass {
 constexpr BatchClass kBatchClass[kBatchClasses] = {PFC_BATCH_CLASSES(PFC_CLASS_ENTRY)};
 #undef PFC_CLASS_ENTRY
 
-// rows per row item and columns per column item of a class
-constexpr int batch_tr(int c) { return kBatchLanes / (kBatchClass[c].n / 2 / kBatchClass[c].p); }
-constexpr int batch_tc(int c) { return kBatchLanes / (kBatchClass[c].n / kBatchClass[c].p); }
-
-#ifndef PFC_BATCH_WAVES
-#define PFC_BATCH_WAVES 2
-#endif
+// Rows per row item and columns per column item: the one-system kernels'
+// tiles (row_cfg, col_cfg), trimmed to at most 1024 lanes per block when a
+// shape has fewer points per thread than those kernels use.
+template <typename T>
+constexpr int batch_tr(int n, int p) {
+    const int tpb = n / 2 / p, tr = row_cfg<T>(n).tr;
+    return tr * tpb <= 1024 ? tr : 1024 / tpb;
+}
+template <typename T>
+constexpr int batch_tc(int n, int p) {
+    const int tpb = n / p, tc = col_cfg(n).tc;
+    return tc * tpb <= 1024 ? tc : 1024 / tpb;
+}
+template <typename T>
+constexpr int batch_tr(int c) { return batch_tr<T>(kBatchClass[c].n, kBatchClass[c].p); }
+template <typename T>
+constexpr int batch_tc(int c) { return batch_tc<T>(kBatchClass[c].n, kBatchClass[c].p); }
 
 // Persistent row pass of one class: the blocks walk the class's items with a
 // grid stride.
 template <typename T, int N, int PP, int U, bool REF>
-__global__ __launch_bounds__(kBatchLanes) PFC_WAVES(PFC_BATCH_WAVES) void k_batch_row(
+__global__ __launch_bounds__(N / 2 / PP * batch_tr<T>(N, PP)) PFC_WAVES(row_waves<T>()) void k_batch_row(
     const BatchItem* __restrict__ items, int nitems, const BatchSys<T>* __restrict__ sys,
-    const int* __restrict__ tdev) {
-    constexpr int TPB = N / 2 / PP, TR = kBatchLanes / TPB;
+    const int* __restrict__ tdev, int toff) {
+    constexpr int TPB = N / 2 / PP, TR = batch_tr<T>(N, PP);
     __shared__ cpx<T> lds[N / 2 * TR];
-    const int tstep = *tdev;
+    const int tstep = *tdev + toff;   // the step's index (NaN reports)
     for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
         const BatchItem it = items[w];
         const BatchSys<T>& S = sys[it.sys];
@@ -88,9 +94,9 @@ __global__ __launch_bounds__(kBatchLanes) PFC_WAVES(PFC_BATCH_WAVES) void k_batc
 
 // Persistent column pass of one class.
 template <typename T, int N, int PP, int U, bool REF>
-__global__ __launch_bounds__(kBatchLanes) PFC_WAVES(PFC_BATCH_WAVES) void k_batch_col(
+__global__ __launch_bounds__(N / PP * batch_tc<T>(N, PP)) PFC_WAVES(col_waves<T>()) void k_batch_col(
     const BatchItem* __restrict__ items, int nitems, const BatchSys<T>* __restrict__ sys) {
-    constexpr int TPB = N / PP, TC = kBatchLanes / TPB;
+    constexpr int TPB = N / PP, TC = batch_tc<T>(N, PP);
     constexpr bool LTAB = col_ltab<T, N, TC>();
     __shared__ cpx<T> lds[N * TC];
     __shared__ __attribute__((aligned(16))) cpx<T> ltw[LTAB ? N : 1];
@@ -105,13 +111,13 @@ __global__ __launch_bounds__(kBatchLanes) PFC_WAVES(PFC_BATCH_WAVES) void k_batc
     }
 }
 
-// advances the device step counter read by the row pass (one per step)
-__global__ void k_batch_tick(int* tdev);
+// advances the device step counter read by the row pass by the steps just run
+__global__ void k_batch_tick(int* tdev, int nsteps);
 
 template <typename T>
 struct BatchLaunch {
     static void row(int cls, bool ref, const BatchItem* items, int nitems, const BatchSys<T>* sys,
-                    const int* tdev, int grid, hipStream_t s);
+                    const int* tdev, int toff, int grid, hipStream_t s);
     static void col(int cls, bool ref, const BatchItem* items, int nitems, const BatchSys<T>* sys,
                     int grid, hipStream_t s);
     // resident blocks per CU of a class's row or column kernel
@@ -125,16 +131,19 @@ struct BatchLaunch {
 #ifdef PFC_BATCH_INSTANTIATE
 template <typename T>
 void BatchLaunch<T>::row(int cls, bool ref, const BatchItem* items, int nitems,
-                         const BatchSys<T>* sys, const int* tdev, int grid, hipStream_t s) {
+                         const BatchSys<T>* sys, const int* tdev, int toff, int grid,
+                         hipStream_t s) {
     switch (cls) {
 #define PFC_LROW(ID, N, PP, U)                                                                   \
     case ID:                                                                                     \
         if (ref)                                                                                 \
-            hipLaunchKernelGGL((k_batch_row<T, N, PP, U, true>), dim3(grid), dim3(kBatchLanes), 0, \
-                               s, items, nitems, sys, tdev);                                     \
+            hipLaunchKernelGGL((k_batch_row<T, N, PP, U, true>), dim3(grid),                     \
+                               dim3(N / 2 / PP * batch_tr<T>(N, PP)), 0, s, items, nitems, sys,  \
+                               tdev, toff);                                                      \
         else                                                                                     \
-            hipLaunchKernelGGL((k_batch_row<T, N, PP, U, false>), dim3(grid), dim3(kBatchLanes), \
-                               0, s, items, nitems, sys, tdev);                                  \
+            hipLaunchKernelGGL((k_batch_row<T, N, PP, U, false>), dim3(grid),                    \
+                               dim3(N / 2 / PP * batch_tr<T>(N, PP)), 0, s, items, nitems, sys,  \
+                               tdev, toff);                                                      \
         break;
         PFC_BATCH_CLASSES(PFC_LROW)
 #undef PFC_LROW
@@ -147,11 +156,11 @@ void BatchLaunch<T>::col(int cls, bool ref, const BatchItem* items, int nitems,
 #define PFC_LCOL(ID, N, PP, U)                                                                   \
     case ID:                                                                                     \
         if (ref)                                                                                 \
-            hipLaunchKernelGGL((k_batch_col<T, N, PP, U, true>), dim3(grid), dim3(kBatchLanes), 0, \
-                               s, items, nitems, sys);                                           \
+            hipLaunchKernelGGL((k_batch_col<T, N, PP, U, true>), dim3(grid),                     \
+                               dim3(N / PP * batch_tc<T>(N, PP)), 0, s, items, nitems, sys);      \
         else                                                                                     \
-            hipLaunchKernelGGL((k_batch_col<T, N, PP, U, false>), dim3(grid), dim3(kBatchLanes), \
-                               0, s, items, nitems, sys);                                        \
+            hipLaunchKernelGGL((k_batch_col<T, N, PP, U, false>), dim3(grid),                    \
+                               dim3(N / PP * batch_tc<T>(N, PP)), 0, s, items, nitems, sys);      \
         break;
         PFC_BATCH_CLASSES(PFC_LCOL)
 #undef PFC_LCOL
@@ -170,9 +179,9 @@ int BatchLaunch<T>::row_occupancy(int cls) {
 #define PFC_LOCC(ID, N, PP, U)                                                                     \
     case ID:                                                                                       \
         e1 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_batch_row<T, N, PP, U, false>,     \
-                                                          kBatchLanes, 0);                         \
+                                                          N / 2 / PP * batch_tr<T>(N, PP), 0);     \
         e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_batch_row<T, N, PP, U, true>,      \
-                                                          kBatchLanes, 0);                         \
+                                                          N / 2 / PP * batch_tr<T>(N, PP), 0);     \
         break;
         PFC_BATCH_CLASSES(PFC_LOCC)
 #undef PFC_LOCC
@@ -187,9 +196,9 @@ int BatchLaunch<T>::col_occupancy(int cls) {
 #define PFC_LOCC(ID, N, PP, U)                                                                     \
     case ID:                                                                                       \
         e1 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_batch_col<T, N, PP, U, false>,     \
-                                                          kBatchLanes, 0);                         \
+                                                          N / PP * batch_tc<T>(N, PP), 0);         \
         e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_batch_col<T, N, PP, U, true>,      \
-                                                          kBatchLanes, 0);                         \
+                                                          N / PP * batch_tc<T>(N, PP), 0);         \
         break;
         PFC_BATCH_CLASSES(PFC_LOCC)
 #undef PFC_LOCC

@@ -66,7 +66,7 @@ def test_fp64_batch_follows_oracle(gpu):
 def candidates(n, U):
     """points per thread the autotuner chooses between"""
     if n & (n - 1) == 0:
-        return (8, 16) if U == 1 or n == 2048 else (8, 4)
+        return (8, 16) if U == 1 else (8, 4)
     return (12, 6) if U == 1 else (6, 12)
 
 
@@ -88,14 +88,27 @@ def test_graph_replays_equal_eager_launches(gpu, precision):
     sub = SYSTEMS[:5]
     a, b = engines(precision, sub), engines(precision, sub)
     with Batch([e for e, *_ in a]) as ga, Batch([e for e, *_ in b], BATCH_NO_GRAPH) as gb:
-        t = 97
-        for n in (2, 3, 1, 4):
+        # single steps, a 20-step block starting with the refresh at 100, a
+        # plain block, and a single refresh step at 200
+        for t, n in ((97, 3), (100, 40), (140, 3), (143, 60)):
             ga.step(t, n)
             gb.step(t, n)
-            t += n
             for (x, *_), (y, *_) in zip(a, b):
                 assert np.array_equal(x.get_fields(), y.get_fields())
                 assert x.evaluate()[0] == y.evaluate()[0]
+
+
+def test_fp64_blocks_match_one_system_engines(gpu):
+    """45 steps from t = 0: two captured 20-step blocks (the first starting
+    with a refresh) and five single steps, against pfc_step on each system."""
+    sub = SYSTEMS[:6]
+    es, ref = engines(FP64, sub), engines(FP64, sub)
+    with Batch([e for e, *_ in es]) as b:
+        b.step(0, 45)
+    for (e, *_), (r, *_), s in zip(es, ref, sub):
+        r.step(0, 45)
+        err = rel_l2(e.get_fields(), r.get_fields())
+        assert err < 1e-12, (s, err)
 
 
 def test_nan_is_reported_per_system(gpu):
