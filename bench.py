@@ -243,11 +243,17 @@ def main(argv=None):
     ap.add_argument("--batch-systems", type=int, default=96,
                     help="systems of the batched-relaxation leg (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=100)
+    ap.add_argument("--batch-only", action="store_true",
+                    help="run only the batched-relaxation leg and print its JSON object")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported "
               "(the whole grid lives on one MI355X; see the sharding notes)", file=sys.stderr)
         return 2
+
+    if a.batch_only:
+        print(json.dumps(batch_bench(a.batch_systems, a.batch_steps, 10)))
+        return 0
 
     from pfc_amd import FP64, MIXED, LATTICE_A, hetero_params
     from pfc_amd.inputs import hetero_patchwork
