@@ -85,7 +85,8 @@ struct pfc_batch {
     double chunk_bytes = 16.0 * (1 << 20);
     bool chunk_fixed = false;
     int* tdev = nullptr;
-    int* tnan = nullptr;   // NaN flag of the tuning passes
+    int* tnan = nullptr;      // NaN flag of the tuning passes
+    double* tn0 = nullptr;    // k = 0 states of the tuning passes (8 doubles per system)
     int thost = 0;
     hipGraph_t graph[4] = {};        // 1 step, 1 refresh step, block, block from a refresh
     hipGraphExec_t exec[4] = {};
@@ -99,7 +100,7 @@ namespace {
 // the context's evaluate buffers R (as I) and O (as Nh) and reports NaNs to a
 // scratch flag, so timing passes change no state.
 template <typename T>
-BatchSys<T> sys_of(const pfc_ctx* c, bool scratch, int* scratch_nan) {
+BatchSys<T> sys_of(const pfc_ctx* c, bool scratch, int* scratch_nan, double* scratch_n0) {
     BatchSys<T> b{};
     b.I = (cpx<T>*)(scratch ? c->R : c->I);
     b.J = (cpx<T>*)c->J;
@@ -115,6 +116,7 @@ BatchSys<T> sys_of(const pfc_ctx* c, bool scratch, int* scratch_nan) {
     b.ktab = (const T*)c->ktab;
     b.prm = c->dprm;
     b.nan = scratch ? scratch_nan : c->nan_flag;
+    b.n0 = scratch ? scratch_n0 : c->n0;
     return b;
 }
 
@@ -360,8 +362,8 @@ template <typename T>
 int build(pfc_batch* b) {
     std::vector<BatchSys<T>> hs(b->n), ht(b->n);
     for (int i = 0; i < b->n; i++) {
-        hs[i] = sys_of<T>(b->ctx[i], false, nullptr);
-        ht[i] = sys_of<T>(b->ctx[i], true, b->tnan);
+        hs[i] = sys_of<T>(b->ctx[i], false, nullptr, nullptr);
+        ht[i] = sys_of<T>(b->ctx[i], true, b->tnan, b->tn0 + 8 * i);
     }
     if (hipMalloc(&b->dsys, b->n * sizeof(BatchSys<T>)) != hipSuccess ||
         hipMalloc(&b->dtune, b->n * sizeof(BatchSys<T>)) != hipSuccess)
@@ -427,6 +429,7 @@ int pfc_batch_destroy(pfc_batch* b) {
     if (b->cols.d) (void)hipFree(b->cols.d);
     if (b->tdev) (void)hipFree(b->tdev);
     if (b->tnan) (void)hipFree(b->tnan);
+    if (b->tn0) (void)hipFree(b->tn0);
     for (int i = 0; i < kBatchStreams; i++) {
         if (b->ss[i]) (void)hipStreamDestroy(b->ss[i]);
         if (b->join[i]) (void)hipEventDestroy(b->join[i]);
@@ -462,7 +465,8 @@ int pfc_batch_create(pfc_batch** out, pfc_ctx* const* ctxs, int n, int flags) {
         hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
         hipMalloc((void**)&b->tdev, sizeof(int)) != hipSuccess ||
-        hipMalloc((void**)&b->tnan, sizeof(int)) != hipSuccess) {
+        hipMalloc((void**)&b->tnan, sizeof(int)) != hipSuccess ||
+        hipMalloc((void**)&b->tn0, 8 * sizeof(double) * n) != hipSuccess) {
         pfc_batch_destroy(b);
         return PFC_EDEVICE;
     }

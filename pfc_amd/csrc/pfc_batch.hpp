@@ -31,6 +31,7 @@ struct BatchSys {
     const T* ktab;
     const DevParams* prm;
     int* nan;
+    double* n0;
 };
 
 // a row group (row pass) or column block (column pass) of one system
@@ -107,7 +108,7 @@ __global__ __launch_bounds__(N / PP * batch_tc<T>(N, PP)) PFC_WAVES(col_waves<T>
         __syncthreads();   // the previous item's last reads of the tables
         col_step_body<T, N, TPB, TC, U, REF, LTAB>(lds, ltw, lgy, it.blk, S.J, S.K, S.Nh, S.I,
                                                    S.fstride, S.nstride, S.H, S.sI, S.sJ, S.twy,
-                                                   S.ktab, S.prm);
+                                                   S.ktab, S.prm, S.n0);
     }
 }
 

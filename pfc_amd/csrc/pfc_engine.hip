@@ -129,7 +129,7 @@ int upload_params(pfc_ctx* c) {
 
 void free_all(pfc_ctx* c) {
     void* bufs[] = {c->I, c->J, c->K, c->Nh, c->O, c->mreal, c->R, c->twx, c->twy,
-                    c->ktab, c->f, c->partials, c->sums, c->nan_flag, c->dprm};
+                    c->ktab, c->f, c->partials, c->sums, c->nan_flag, c->dprm, c->n0};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     free(c->hktab);
@@ -243,6 +243,7 @@ int pfc_create(pfc_ctx** out, const pfc_params* p, int Nx, int Ny, int precision
     rc |= dalloc((void**)&c->sums, (1 + c->U) * sizeof(double));
     rc |= dalloc((void**)&c->nan_flag, sizeof(int));
     rc |= dalloc((void**)&c->dprm, sizeof(DevParams));
+    rc |= dalloc((void**)&c->n0, 2 * c->U * sizeof(double));
 #if PFC_PROBE
     rc |= dalloc((void**)&c->hp.probe, 2 * 65536 * sizeof(unsigned long long));
 #endif
@@ -315,6 +316,16 @@ int pfc_set_fields(pfc_ctx* c, const double* fields) {
         HIPCHK(hipStreamSynchronize(c->stream));
         Launch<float>::row_r2c_any(c, c->mreal, c->I, c->sI, c->U);
         Launch<float>::col_fwd_any(c, c->I, c->sI, c->Nh, c->U, 1.0 / c->Ny);  // I carries 1/Nx
+        // the k = 0 state in FP64: each field's mean, summed in long double
+        std::vector<double> m0(2 * c->U, 0.0);
+        for (int u = 0; u < c->U; u++) {
+            long double acc = 0.0L;
+            const double* f = fields + (size_t)u * c->nr;
+            for (size_t i = 0; i < c->nr; i++) acc += f[i];
+            m0[2 * u] = (double)(acc / (long double)c->nr);
+        }
+        HIPCHK(hipMemcpyAsync(c->n0, m0.data(), m0.size() * sizeof(double), hipMemcpyHostToDevice,
+                              c->stream));
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));

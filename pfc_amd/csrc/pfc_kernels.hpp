@@ -578,7 +578,7 @@ __device__ __forceinline__ void col_step_body(
     cpx<T>* lds, cpx<T>* ltw, T* lgy, int cb, const cpx<T>* __restrict__ J,
     const cpx<T>* __restrict__ K, cpx<T>* __restrict__ Nh, cpx<T>* __restrict__ I, size_t fstride,
     size_t nstride, int H, int sI, int sJ, const cpx<T>* __restrict__ twy_,
-    const T* __restrict__ ktab_, const DevParams* __restrict__ prm) {
+    const T* __restrict__ ktab_, const DevParams* __restrict__ prm, double* __restrict__ n0) {
     constexpr int P = NY / TPB;
     constexpr unsigned E = sizeof(cpx<T>);
     using FFTF = BlockFFT<T, NY, TPB, -1, 1, 1>;
@@ -681,7 +681,29 @@ __device__ __forceinline__ void col_step_body(
                     cpx<T> Mk[U];
 #pragma unroll
                     for (int u = 0; u < U; u++) Mk[u] = M[u][q];
-                    kupdate<U, T>(prm, k2, G2, Mk, Nb[i]);
+                    bool k0 = false;
+                    if constexpr (sizeof(T) == 4) k0 = (jb.mode == 1 && j == 0);
+                    if (k0) {
+                        // k = 0 (the mean density) is carried in FP64: its
+                        // update runs in FP64 from the FP64 state n0, and the
+                        // FP32 state takes the rounded result.  Refreshes from
+                        // the FP32 density leave n0 alone.
+                        cpx<double> M0[U], N0[U];
+#pragma unroll
+                        for (int u = 0; u < U; u++) {
+                            M0[u] = {(double)Mk[u].x, (double)Mk[u].y};
+                            N0[u] = {n0[2 * u], n0[2 * u + 1]};
+                        }
+                        kupdate<U, double>(prm, 0.0, 1.0, M0, N0);
+#pragma unroll
+                        for (int u = 0; u < U; u++) {
+                            n0[2 * u] = N0[u].x;
+                            n0[2 * u + 1] = N0[u].y;
+                            Nb[i][u] = {(T)N0[u].x, (T)N0[u].y};
+                        }
+                    } else {
+                        kupdate<U, T>(prm, k2, G2, Mk, Nb[i]);
+                    }
 #pragma unroll
                     for (int u = 0; u < U; u++) {
                         M[u][q] = Nb[i][u];
@@ -730,13 +752,13 @@ __global__ __launch_bounds__(TPB* TC) PFC_WAVES(col_waves<T>()) void k_col_step(
     const cpx<T>* __restrict__ J, const cpx<T>* __restrict__ K, cpx<T>* __restrict__ Nh,
     cpx<T>* __restrict__ I, size_t fstride, size_t nstride, int H, int sI, int sJ,
     const cpx<T>* __restrict__ twy_, const T* __restrict__ ktab_,
-    const DevParams* __restrict__ prm) {
+    const DevParams* __restrict__ prm, double* __restrict__ n0) {
     constexpr bool LTAB = col_ltab<T, NY, TC>();
     __shared__ cpx<T> lds[NY * TC];
     __shared__ __attribute__((aligned(16))) cpx<T> ltw[LTAB ? NY : 1];   // e^{-2 pi i m/Ny}
     __shared__ __attribute__((aligned(16))) T lgy[LTAB ? NY : 1];        // exp(-ky^2/sigma^2)
     col_step_body<T, NY, TPB, TC, U, REFRESH, LTAB>(lds, ltw, lgy, blockIdx.x, J, K, Nh, I, fstride,
-                                                    nstride, H, sI, sJ, twy_, ktab_, prm);
+                                                    nstride, H, sI, sJ, twy_, ktab_, prm, n0);
 }
 
 // ------------------------------------------------ single-buffer row passes

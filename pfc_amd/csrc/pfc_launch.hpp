@@ -114,6 +114,7 @@ struct pfc_ctx {
     pfc::DevParams hp;
     pfc::DevParams* dprm;
     void *I, *J, *K, *Nh, *O, *mreal, *R, *twx, *twy;
+    double* n0;   // mixed form: the k = 0 state of each field in FP64 (re, im)
     void* ktab;   // Gaussian factors exp(-k^2/sigma^2) per kx and per ky (ktab_gy), set by pfc_set_box
     void* hktab;  // host staging of ktab
     double *f, *partials, *sums;
@@ -182,7 +183,7 @@ struct Launch {
             hipLaunchKernelGGL((k_col_step<T, NY, g.tpb, g.tc, U, REF>), dim3(H / g.tc + 1),
                                dim3(g.tpb * g.tc), 0, c->stream, (const C*)c->J, (const C*)c->K,
                                (C*)c->Nh, (C*)c->I, c->nk, c->nkn, H, c->sI, c->sJ, (const C*)c->twy,
-                               (const T*)c->ktab, c->dprm);
+                               (const T*)c->ktab, c->dprm, c->n0);
         }
     }
     template <int NY, int U>

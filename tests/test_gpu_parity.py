@@ -87,6 +87,22 @@ def test_three_times_power_of_two_grids(gpu, Nx, Ny, precision):
     assert abs(fg - fo) <= (1e-12 if precision == FP64 else 1e-6) * abs(fo), (fg, fo)
 
 
+def test_mixed_k0_mode_is_fp64(gpu):
+    """The mixed form carries the k = 0 mode in FP64.  A uniform field of a
+    linear non-conserved model (gamma = delta = lambda = 0) decays as
+    n <- A0 n with A0 = exp(-(alpha + beta) dt) = exp(-0.001): after 1000 steps
+    the mean must be 0.3 A0^1000 to the FP32 resolution of the output; an FP32
+    recursion would drift by ~1000 roundings of A0 (~1e-5)."""
+    p = graphene_params(alpha=-0.999, gamma=0.0, delta=0.0, c=0.0, dt=1.0)
+    a = -0.999 + 1.0
+    with Engine(p, 64, 64, DX, DX, MIXED) as g:
+        g.set_fields(np.full((1, 64, 64), 0.3))
+        g.step(1, 1000)
+        mean = float(g.get_fields().mean())
+    expected = 0.3 * math.exp(-a) ** 1000
+    assert abs(mean - expected) <= 2e-7 * expected, (mean, expected)
+
+
 @pytest.mark.parametrize("c", [0.0, 1.0])
 def test_graphene_steps_fp64(gpu, c):
     Nx, Ny = 128, 128
