@@ -1,0 +1,83 @@
+"""The measured configurations at their full sizes on the GPU: the 2048^2
+graphene polycrystal in FP64 and the 8192^2 three-field graphene/hBN
+heterostructure in both forms.  Each runs a few steps against the CPU oracle
+(which is too slow for the full runs) and then longer through properties
+that do not depend on the size: conservation of the mean, decrease of the
+free energy, and the mixed form's distance from FP64."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle_lib import Oracle
+from pfc_amd import FP64, MIXED, LATTICE_A, graphene_params, hetero_params
+from pfc_amd.engine import Engine
+from pfc_amd.inputs import hetero_patchwork, polycrystal
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def test_polycrystal_2048_fp64(gpu):
+    """2048^2, 64 seeded grains at random orientations with 2-site boundaries
+    at the mean density, conserved dynamics, dt = 1."""
+    N = 2048
+    dx = LATTICE_A / 8.0
+    p = graphene_params(c=1.0, dt=1.0)
+    f = polycrystal(N, N, dx, dx, 64, 0.2, -0.2, LATTICE_A, sym=3, noise=0.01, boundary=2,
+                    seed=64)[None]
+    o = Oracle(p, N, N, dx, dx)
+    o.set_fields(f)
+    assert o.steps(0, 3) == 0
+    with Engine(p, N, N, dx, dx, FP64) as g:
+        g.set_fields(f)
+        g.step(0, 3)
+        assert rel_l2(g.get_fields(), o.get_fields()) < 1e-12
+        fo, no = o.evaluate()
+        fg, ng = g.evaluate()
+        assert abs(fg - fo) <= 1e-12 * abs(fo)
+        energies = [fg]
+        for t in (3, 103, 203):
+            g.step(t, 100)
+            fe, ne = g.evaluate()
+            energies.append(fe)
+            assert abs(ne[0] - ng[0]) <= 1e-13 * abs(ng[0])      # the mean is conserved
+        assert all(b < a for a, b in zip(energies, energies[1:])), energies
+        assert g.nan_step() == -1
+
+
+def test_heterostructure_8192(gpu):
+    """8192^2, three fields, 128 seeded domains, dt = 0.25: FP64 and mixed
+    against the oracle for 2 steps, then mixed against FP64 over 100 steps
+    (energy per area within 1e-6, the north star's bound).  The hetero.in
+    parameters diverge at dt = 0.5 (the CPU oracle reaches NaN within 20
+    steps at 256^2), so the demo's dt = 0.25 is used."""
+    N = 8192
+    p = hetero_params(dt=0.25, c=1.0, sigma=2.0 / 7.2552)
+    f = hetero_patchwork(N, N, 0.7, 0.7, n_domains=128, seed=8192, noise=0.01)
+    o = Oracle(p, N, N, 0.7, 0.7)
+    o.set_fields(f)
+    assert o.steps(0, 2) == 0
+    ref = o.get_fields()
+    fo, _ = o.evaluate()
+    del o
+    with Engine(p, N, N, 0.7, 0.7, FP64) as d, Engine(p, N, N, 0.7, 0.7, MIXED) as m:
+        d.set_fields(f)
+        m.set_fields(f)
+        d.step(0, 2)
+        m.step(0, 2)
+        assert rel_l2(d.get_fields(), ref) < 1e-12
+        assert rel_l2(m.get_fields(), ref) < 2e-6
+        fd, nd = d.evaluate()
+        assert abs(fd - fo) <= 1e-12 * abs(fo)
+        d.step(2, 98)
+        m.step(2, 98)
+        fd, nd = d.evaluate()
+        fm, nm = m.evaluate()
+        assert abs(fm - fd) <= 1e-6 * abs(fd), (fm, fd)
+        assert np.allclose(nm, nd, rtol=1e-7, atol=0)
+        assert rel_l2(m.get_fields(), d.get_fields()) < 1e-3
+        assert d.nan_step() == -1 and m.nan_step() == -1
