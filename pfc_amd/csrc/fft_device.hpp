@@ -171,35 +171,34 @@ __device__ __forceinline__ cpx<float> gload<float>(GBuf b, unsigned voff, unsign
     const auto v = __builtin_amdgcn_raw_buffer_load_b64(b.r, (int)voff, (int)soff, 0);
     return __builtin_bit_cast(cpx<float>, v);
 }
-template <typename T>
-__device__ __forceinline__ void gstore(GBuf b, unsigned voff, unsigned soff, cpx<T> x);
-template <>
-__device__ __forceinline__ void gstore<double>(GBuf b, unsigned voff, unsigned soff,
-                                               cpx<double> x) {
+// Stores a complex element.  AUX is the buffer instruction's cache-policy
+// operand (gfx950: bit 1 = nt, non-temporal, for data the same pass does not
+// read again).
+template <typename T, int AUX = 0>
+__device__ __forceinline__ void gstore(GBuf b, unsigned voff, unsigned soff, cpx<T> x) {
+    using V = decltype(__builtin_amdgcn_raw_buffer_load_b64(b.r, 0, 0, 0));
+    if constexpr (sizeof(T) == 8) {
 #if PFC_STORE_B128 == 1
-    using V = decltype(__builtin_amdgcn_raw_buffer_load_b128(b.r, 0, 0, 0));
-    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(V, x), b.r, (int)voff, (int)soff, 0);
+        using W = decltype(__builtin_amdgcn_raw_buffer_load_b128(b.r, 0, 0, 0));
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(W, x), b.r, (int)voff, (int)soff, AUX);
 #elif PFC_STORE_B128 == 2
-    using V = decltype(__builtin_amdgcn_raw_buffer_load_b128(b.r, 0, 0, 0));
-    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(V, x), b.r, (int)(voff + soff), 0, 0);
+        using W = decltype(__builtin_amdgcn_raw_buffer_load_b128(b.r, 0, 0, 0));
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(W, x), b.r, (int)(voff + soff), 0, AUX);
 #else
-    // Two 64-bit stores of the same bytes instead of one 128-bit store: a
-    // 128-bit buffer store with an SGPR soffset whose data VGPRs are rewritten
-    // by the next VALU instructions has stored corrupted upper halves on
-    // gfx950 (the compiler inserts no wait states for that case).  The second
-    // half's address goes through an opaque register copy so the backend
-    // cannot merge the pair back into a 128-bit store.
-    using V = decltype(__builtin_amdgcn_raw_buffer_load_b64(b.r, 0, 0, 0));
-    unsigned v2 = voff + 8u;
-    asm volatile("" : "+v"(v2));
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V, x.x), b.r, (int)voff, (int)soff, 0);
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V, x.y), b.r, (int)v2, (int)soff, 0);
+        // Two 64-bit stores of the same bytes instead of one 128-bit store: a
+        // 128-bit buffer store with an SGPR soffset whose data VGPRs are
+        // rewritten by the next VALU instructions has stored corrupted upper
+        // halves on gfx950 (the compiler inserts no wait states for that
+        // case).  The second half's address goes through an opaque register
+        // copy so the backend cannot merge the pair back into a 128-bit store.
+        unsigned v2 = voff + 8u;
+        asm volatile("" : "+v"(v2));
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V, x.x), b.r, (int)voff, (int)soff, AUX);
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V, x.y), b.r, (int)v2, (int)soff, AUX);
 #endif
-}
-template <>
-__device__ __forceinline__ void gstore<float>(GBuf b, unsigned voff, unsigned soff, cpx<float> x) {
-    using V = decltype(__builtin_amdgcn_raw_buffer_load_b64(b.r, 0, 0, 0));
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V, x), b.r, (int)voff, (int)soff, 0);
+    } else {
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V, x), b.r, (int)voff, (int)soff, AUX);
+    }
 }
 
 // one real value
@@ -215,16 +214,14 @@ __device__ __forceinline__ float gload_real<float>(GBuf b, unsigned voff) {
 }
 
 // one real value (the real or imaginary half of a complex element)
-template <typename T>
-__device__ __forceinline__ void gstore_real(GBuf b, unsigned voff, unsigned soff, T x);
-template <>
-__device__ __forceinline__ void gstore_real<double>(GBuf b, unsigned voff, unsigned soff, double x) {
-    using V = decltype(__builtin_amdgcn_raw_buffer_load_b64(b.r, 0, 0, 0));
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V, x), b.r, (int)voff, (int)soff, 0);
-}
-template <>
-__device__ __forceinline__ void gstore_real<float>(GBuf b, unsigned voff, unsigned soff, float x) {
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, x), b.r, (int)voff, (int)soff, 0);
+template <typename T, int AUX = 0>
+__device__ __forceinline__ void gstore_real(GBuf b, unsigned voff, unsigned soff, T x) {
+    if constexpr (sizeof(T) == 8) {
+        using V = decltype(__builtin_amdgcn_raw_buffer_load_b64(b.r, 0, 0, 0));
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V, x), b.r, (int)voff, (int)soff, AUX);
+    } else {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, x), b.r, (int)voff, (int)soff, AUX);
+    }
 }
 
 constexpr int ilog(int b, int n) { return n < b ? 0 : 1 + ilog(b, n / b); }

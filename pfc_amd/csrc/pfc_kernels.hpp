@@ -30,6 +30,18 @@ constexpr int kMaxU = 4;   // fields supported by the fused kernels
 #ifndef PFC_PROBE
 #define PFC_PROBE 0
 #endif
+// Cache policy of the step passes' stores (0, or 2 = non-temporal).  The
+// column pass writes whole columns that no later block of the pass reads, so
+// its stores stream past L2 (1-2 % faster step).  The row pass's 32-byte
+// pieces of J must stay cacheable: L2 merges them with the neighbouring row
+// groups' pieces of the same lines; non-temporal they are 1.6x (mixed) to
+// 2.4x (FP64) slower.
+#ifndef PFC_ROWJ_AUX
+#define PFC_ROWJ_AUX 0
+#endif
+#ifndef PFC_COL_AUX
+#define PFC_COL_AUX 2
+#endif
 #ifndef PFC_ROW_WAVES_F64
 #define PFC_ROW_WAVES_F64 2
 #endif
@@ -487,7 +499,7 @@ __device__ __forceinline__ void row_step_body(
         r2c_post<T, H, TPB, TR>(m[u], L, row_wt<T>(twx, t), t);
         const GBuf dst = gbuf(J + u * fstride);
 #pragma unroll
-        for (int q = 0; q < P; q++) gstore<T>(dst, voff, q * qstep, m[u][q]);
+        for (int q = 0; q < P; q++) gstore<T, PFC_ROWJ_AUX>(dst, voff, q * qstep, m[u][q]);
     }
     PFC_PROBE_AT(0, 4);
 }
@@ -498,8 +510,12 @@ __global__ __launch_bounds__(TPB* TR) PFC_WAVES(row_waves<T>()) void k_row_step(
     size_t fstride, int sI, int sJ, const cpx<T>* __restrict__ twx_,
     const DevParams* __restrict__ prm, int* __restrict__ nan_step, int tstep) {
     __shared__ cpx<T> lds[NX / 2 * TR];
-    row_step_body<T, NX, TPB, TR, U, REFRESH>(lds, xcd_remap(blockIdx.x, gridDim.x), I, J, K,
-                                              fstride, sI, sJ, twx_, prm, nan_step, tstep);
+#ifndef PFC_ROW_XCD
+#define PFC_ROW_XCD 1
+#endif
+    const int g = PFC_ROW_XCD ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    row_step_body<T, NX, TPB, TR, U, REFRESH>(lds, g, I, J, K, fstride, sI, sJ, twx_, prm,
+                                              nan_step, tstep);
 }
 
 // Column addressing modes of the column passes.  mode 0: an ordinary column
@@ -513,10 +529,10 @@ __device__ __forceinline__ cpx<T> col_in(GBuf b, unsigned voff, unsigned soff, i
     if (mode == 0) return v;
     return {mode == 1 ? v.x : v.y, (T)0};
 }
-template <typename T>
+template <typename T, int AUX = 0>
 __device__ __forceinline__ void col_out(GBuf b, unsigned voff, unsigned soff, int mode, cpx<T> v) {
-    if (mode == 0) gstore<T>(b, voff, soff, v);
-    else gstore_real<T>(b, voff + (mode == 2 ? (unsigned)sizeof(T) : 0u), soff, v.x);
+    if (mode == 0) gstore<T, AUX>(b, voff, soff, v);
+    else gstore_real<T, AUX>(b, voff + (mode == 2 ? (unsigned)sizeof(T) : 0u), soff, v.x);
 }
 
 // Which column lane group c of block b handles.  The grid has H/TC + 1
@@ -707,7 +723,7 @@ __device__ __forceinline__ void col_step_body(
 #pragma unroll
                     for (int u = 0; u < U; u++) {
                         M[u][q] = Nb[i][u];
-                        gstore<T>(gbuf(Nh + u * nstride), vn, q * TPB * E, M[u][q]);
+                        gstore<T, PFC_COL_AUX>(gbuf(Nh + u * nstride), vn, q * TPB * E, M[u][q]);
                     }
                 }
             }
@@ -733,7 +749,8 @@ __device__ __forceinline__ void col_step_body(
             if (jb.live) {
                 const GBuf dst = gbuf(I + u * fstride);
 #pragma unroll
-                for (int q = 0; q < P; q++) col_out<T>(dst, vi, q * istep, jb.mode, M[u][q]);
+                for (int q = 0; q < P; q++)
+                    col_out<T, PFC_COL_AUX>(dst, vi, q * istep, jb.mode, M[u][q]);
             }
         }
     }
