@@ -159,8 +159,8 @@ int pfc_fields(const double* density, int W, int H, double dx, double dy, int fo
  * pfc_evaluate, pfc_set_box, pfc_nan_step); pfc_batch_step advances all of
  * them by steps t0 ... t0 + nsteps - 1 exactly as pfc_step would.  Systems of
  * equal (Nx, Ny, U) are cut into chunks of a cache-sized working set; each
- * chunk's row and column passes are persistent launches over all its
- * systems' work, chained on one of four streams, and unless
+ * chunk's row and column passes are launches of one block per row group or
+ * column block of all its systems, chained on one of four streams, and unless
  * PFC_BATCH_NO_GRAPH, blocks of 20 steps (single steps where t0 or nsteps do
  * not align) are replayed from captured HIP graphs.  PFC_BATCH_AUTOTUNE times both
  * FFT shapes of every (length, U) and pass present -- points per thread 8 or

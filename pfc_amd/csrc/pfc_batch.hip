@@ -1,7 +1,7 @@
 // pfc_batch.hip -- the pfc_batch_* C ABI: many independent systems of uneven
 // sizes stepped together.  Systems of one (Nx, Ny, U) are cut into chunks of a
-// cache-sized working set; each chunk's passes are persistent launches over
-// all its systems' work items (pfc_batch.hpp), chained step after step on one
+// cache-sized working set; each chunk's passes are launches of one block per
+// work item of all its systems (pfc_batch.hpp), chained step after step on one
 // of a few streams; blocks of steps are captured as HIP graphs; a small
 // autotuner picks the FFT shape of every (length, U) and the chunk budget.
 //
@@ -10,7 +10,9 @@
 // every pass ran 1.88 ms per step; four streams 1.65; the same systems stepped
 // one by one with a join per step 1.65 and unsynchronised 1.49; 20-step graph
 // blocks and 16 MiB chunks (the row transforms of a chunk are still in the
-// Infinity Cache when its column pass reads them) bring the batch to 1.55.
+// Infinity Cache when its column pass reads them) bring the batch to 1.55;
+// one block per work item instead of persistent blocks walking the items with
+// a grid stride, 1.50 against 1.67 on the same box.
 //
 // A batch borrows existing contexts: their buffers, parameters, box tables and
 // NaN flags.  The contexts stay usable through the one-system calls (fields
@@ -57,10 +59,10 @@ std::map<std::tuple<int, int, int, int>, int>& tuned() {
 
 using namespace pfc;
 
-// The launches of one pass: per class present, its items' offset, count and
-// persistent grid, largest classes first.
+// The launches of one pass: per chunk, its class and its items' offset and
+// count (one block per item), largest chunks first.
 struct BatchPass {
-    std::vector<int> cls, off, cnt, grid;
+    std::vector<int> cls, off, cnt;
     std::vector<double> work;
     BatchItem* d = nullptr;
 };
@@ -91,7 +93,6 @@ struct pfc_batch {
     hipGraph_t graph[4] = {};        // 1 step, 1 refresh step, block, block from a refresh
     hipGraphExec_t exec[4] = {};
     std::map<std::pair<int, int>, int> rclass, cclass;   // (length, U) -> class
-    int cus = 256;
 };
 
 namespace {
@@ -179,12 +180,9 @@ int build_pass(const pfc_batch* b, bool row_pass, const std::map<std::pair<int, 
             for (int w = 0; w < n8; w++) q[w] = v[(w % 8) * (n8 / 8) + w / 8];
             v.swap(q);
         }
-        int g = std::min((int)v.size(), BatchLaunch<T>::occupancy(cls, row_pass) * b->cus);
-        if (g >= 8) g -= g % 8;
         p.cls.push_back(cls);
         p.off.push_back((int)all.size());
         p.cnt.push_back((int)v.size());
-        p.grid.push_back(std::max(g, 1));
         p.work.push_back(og.first);
         all.insert(all.end(), v.begin(), v.end());
     }
@@ -204,10 +202,9 @@ void launch_entry(const pfc_batch* b, bool row_pass, const BatchPass& p, int k, 
                   const void* sys, hipStream_t st, int toff = 0) {
     if (row_pass)
         BatchLaunch<T>::row(p.cls[k], ref, p.d + p.off[k], p.cnt[k], (const BatchSys<T>*)sys,
-                            b->tdev, toff, p.grid[k], st);
+                            b->tdev, toff, st);
     else
-        BatchLaunch<T>::col(p.cls[k], ref, p.d + p.off[k], p.cnt[k], (const BatchSys<T>*)sys,
-                            p.grid[k], st);
+        BatchLaunch<T>::col(p.cls[k], ref, p.d + p.off[k], p.cnt[k], (const BatchSys<T>*)sys, st);
 }
 
 // Forks the first ns batch streams from the batch stream / joins them back.
@@ -459,10 +456,7 @@ int pfc_batch_create(pfc_batch** out, pfc_ctx* const* ctxs, int n, int flags) {
         b->chunk_bytes = atof(e) * (1 << 20);
         b->chunk_fixed = true;
     }
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking) != hipSuccess ||
+    if (hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
         hipMalloc((void**)&b->tdev, sizeof(int)) != hipSuccess ||
         hipMalloc((void**)&b->tnan, sizeof(int)) != hipSuccess ||

@@ -2,10 +2,11 @@
 // sizes (pfc_batch_* in include/pfc_relax.h).
 //
 // Systems are grouped into classes (FFT length, points per thread P, fields
-// U).  Each pass of a step launches one persistent kernel per group of
-// systems of one class: its blocks walk the group's work items (a row group,
-// or a column block, of one of its systems) with a grid stride, running the
-// same row and column bodies and tiles as the one-system kernels.  P sets the
+// U).  Each pass of a step launches one kernel per chunk of systems of one
+// class, with one block per work item (a row group, or a column block, of one
+// of its systems), running the same row and column bodies and tiles as the
+// one-system kernels.  (Persistent blocks walking the items with a grid
+// stride ran the batched-relaxation workload 10 % slower.)  P sets the
 // FFT radix (8, 16 or 4 for 2^k; 12 or 6 for 3 * 2^k) and with it the lanes
 // per transform; the autotuner picks P per (length, U) and pass by timing
 // both candidates.  (One kernel holding every class, dispatching per
@@ -76,8 +77,7 @@ constexpr int batch_tr(int c) { return batch_tr<T>(kBatchClass[c].n, kBatchClass
 template <typename T>
 constexpr int batch_tc(int c) { return batch_tc<T>(kBatchClass[c].n, kBatchClass[c].p); }
 
-// Persistent row pass of one class: the blocks walk the class's items with a
-// grid stride.
+// Row pass of one class: block w runs item w.
 template <typename T, int N, int PP, int U, bool REF>
 __global__ __launch_bounds__(N / 2 / PP * batch_tr<T>(N, PP)) PFC_WAVES(row_waves<T>()) void k_batch_row(
     const BatchItem* __restrict__ items, int nitems, const BatchSys<T>* __restrict__ sys,
@@ -85,15 +85,14 @@ __global__ __launch_bounds__(N / 2 / PP * batch_tr<T>(N, PP)) PFC_WAVES(row_wave
     constexpr int TPB = N / 2 / PP, TR = batch_tr<T>(N, PP);
     __shared__ cpx<T> lds[N / 2 * TR];
     const int tstep = *tdev + toff;   // the step's index (NaN reports)
-    for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
-        const BatchItem it = items[w];
-        const BatchSys<T>& S = sys[it.sys];
-        row_step_body<T, N, TPB, TR, U, REF>(lds, it.blk, S.I, S.J, S.K, S.fstride, S.sI, S.sJ,
-                                             S.twx, S.prm, S.nan, tstep);
-    }
+    if ((int)blockIdx.x >= nitems) return;
+    const BatchItem it = items[blockIdx.x];
+    const BatchSys<T>& S = sys[it.sys];
+    row_step_body<T, N, TPB, TR, U, REF>(lds, it.blk, S.I, S.J, S.K, S.fstride, S.sI, S.sJ, S.twx,
+                                         S.prm, S.nan, tstep);
 }
 
-// Persistent column pass of one class.
+// Column pass of one class: block w runs item w.
 template <typename T, int N, int PP, int U, bool REF>
 __global__ __launch_bounds__(N / PP * batch_tc<T>(N, PP)) PFC_WAVES(col_waves<T>()) void k_batch_col(
     const BatchItem* __restrict__ items, int nitems, const BatchSys<T>* __restrict__ sys) {
@@ -102,14 +101,12 @@ __global__ __launch_bounds__(N / PP * batch_tc<T>(N, PP)) PFC_WAVES(col_waves<T>
     __shared__ cpx<T> lds[N * TC];
     __shared__ __attribute__((aligned(16))) cpx<T> ltw[LTAB ? N : 1];
     __shared__ __attribute__((aligned(16))) T lgy[LTAB ? N : 1];
-    for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
-        const BatchItem it = items[w];
-        const BatchSys<T>& S = sys[it.sys];
-        __syncthreads();   // the previous item's last reads of the tables
-        col_step_body<T, N, TPB, TC, U, REF, LTAB>(lds, ltw, lgy, it.blk, S.J, S.K, S.Nh, S.I,
-                                                   S.fstride, S.nstride, S.H, S.sI, S.sJ, S.twy,
-                                                   S.ktab, S.prm, S.n0);
-    }
+    if ((int)blockIdx.x >= nitems) return;
+    const BatchItem it = items[blockIdx.x];
+    const BatchSys<T>& S = sys[it.sys];
+    col_step_body<T, N, TPB, TC, U, REF, LTAB>(lds, ltw, lgy, it.blk, S.J, S.K, S.Nh, S.I, S.fstride,
+                                               S.nstride, S.H, S.sI, S.sJ, S.twy, S.ktab, S.prm,
+                                               S.n0);
 }
 
 // advances the device step counter read by the row pass by the steps just run
@@ -117,14 +114,11 @@ __global__ void k_batch_tick(int* tdev, int nsteps);
 
 template <typename T>
 struct BatchLaunch {
+    // one block per item
     static void row(int cls, bool ref, const BatchItem* items, int nitems, const BatchSys<T>* sys,
-                    const int* tdev, int toff, int grid, hipStream_t s);
+                    const int* tdev, int toff, hipStream_t s);
     static void col(int cls, bool ref, const BatchItem* items, int nitems, const BatchSys<T>* sys,
-                    int grid, hipStream_t s);
-    // resident blocks per CU of a class's row or column kernel
-    static int row_occupancy(int cls);
-    static int col_occupancy(int cls);
-    static int occupancy(int cls, bool row_pass) { return row_pass ? row_occupancy(cls) : col_occupancy(cls); }
+                    hipStream_t s);
 };
 
 // Out-of-line definitions, compiled only where BatchLaunch's members are
@@ -132,19 +126,18 @@ struct BatchLaunch {
 #ifdef PFC_BATCH_INSTANTIATE
 template <typename T>
 void BatchLaunch<T>::row(int cls, bool ref, const BatchItem* items, int nitems,
-                         const BatchSys<T>* sys, const int* tdev, int toff, int grid,
-                         hipStream_t s) {
+                         const BatchSys<T>* sys, const int* tdev, int toff, hipStream_t s) {
     switch (cls) {
-#define PFC_LROW(ID, N, PP, U)                                                                   \
-    case ID:                                                                                     \
-        if (ref)                                                                                 \
-            hipLaunchKernelGGL((k_batch_row<T, N, PP, U, true>), dim3(grid),                     \
-                               dim3(N / 2 / PP * batch_tr<T>(N, PP)), 0, s, items, nitems, sys,  \
-                               tdev, toff);                                                      \
-        else                                                                                     \
-            hipLaunchKernelGGL((k_batch_row<T, N, PP, U, false>), dim3(grid),                    \
-                               dim3(N / 2 / PP * batch_tr<T>(N, PP)), 0, s, items, nitems, sys,  \
-                               tdev, toff);                                                      \
+#define PFC_LROW(ID, N, PP, U)                                                                     \
+    case ID:                                                                                       \
+        if (ref)                                                                                   \
+            hipLaunchKernelGGL((k_batch_row<T, N, PP, U, true>), dim3(nitems),                     \
+                               dim3(N / 2 / PP * batch_tr<T>(N, PP)), 0, s, items, nitems, sys,    \
+                               tdev, toff);                                                        \
+        else                                                                                       \
+            hipLaunchKernelGGL((k_batch_row<T, N, PP, U, false>), dim3(nitems),                    \
+                               dim3(N / 2 / PP * batch_tr<T>(N, PP)), 0, s, items, nitems, sys,    \
+                               tdev, toff);                                                        \
         break;
         PFC_BATCH_CLASSES(PFC_LROW)
 #undef PFC_LROW
@@ -152,59 +145,20 @@ void BatchLaunch<T>::row(int cls, bool ref, const BatchItem* items, int nitems,
 }
 template <typename T>
 void BatchLaunch<T>::col(int cls, bool ref, const BatchItem* items, int nitems,
-                         const BatchSys<T>* sys, int grid, hipStream_t s) {
+                         const BatchSys<T>* sys, hipStream_t s) {
     switch (cls) {
-#define PFC_LCOL(ID, N, PP, U)                                                                   \
-    case ID:                                                                                     \
-        if (ref)                                                                                 \
-            hipLaunchKernelGGL((k_batch_col<T, N, PP, U, true>), dim3(grid),                     \
-                               dim3(N / PP * batch_tc<T>(N, PP)), 0, s, items, nitems, sys);      \
-        else                                                                                     \
-            hipLaunchKernelGGL((k_batch_col<T, N, PP, U, false>), dim3(grid),                    \
-                               dim3(N / PP * batch_tc<T>(N, PP)), 0, s, items, nitems, sys);      \
+#define PFC_LCOL(ID, N, PP, U)                                                                     \
+    case ID:                                                                                       \
+        if (ref)                                                                                   \
+            hipLaunchKernelGGL((k_batch_col<T, N, PP, U, true>), dim3(nitems),                     \
+                               dim3(N / PP * batch_tc<T>(N, PP)), 0, s, items, nitems, sys);       \
+        else                                                                                       \
+            hipLaunchKernelGGL((k_batch_col<T, N, PP, U, false>), dim3(nitems),                    \
+                               dim3(N / PP * batch_tc<T>(N, PP)), 0, s, items, nitems, sys);       \
         break;
         PFC_BATCH_CLASSES(PFC_LCOL)
 #undef PFC_LCOL
     }
-}
-inline int lower_positive(int a, int b, hipError_t e1, hipError_t e2) {
-    if (e1 != hipSuccess || e2 != hipSuccess) return 1;
-    const int m = a < b ? a : b;
-    return m > 0 ? m : 1;
-}
-template <typename T>
-int BatchLaunch<T>::row_occupancy(int cls) {
-    int a = 0, b = 0;
-    hipError_t e1 = hipErrorInvalidValue, e2 = hipErrorInvalidValue;
-    switch (cls) {
-#define PFC_LOCC(ID, N, PP, U)                                                                     \
-    case ID:                                                                                       \
-        e1 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_batch_row<T, N, PP, U, false>,     \
-                                                          N / 2 / PP * batch_tr<T>(N, PP), 0);     \
-        e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_batch_row<T, N, PP, U, true>,      \
-                                                          N / 2 / PP * batch_tr<T>(N, PP), 0);     \
-        break;
-        PFC_BATCH_CLASSES(PFC_LOCC)
-#undef PFC_LOCC
-    }
-    return lower_positive(a, b, e1, e2);
-}
-template <typename T>
-int BatchLaunch<T>::col_occupancy(int cls) {
-    int a = 0, b = 0;
-    hipError_t e1 = hipErrorInvalidValue, e2 = hipErrorInvalidValue;
-    switch (cls) {
-#define PFC_LOCC(ID, N, PP, U)                                                                     \
-    case ID:                                                                                       \
-        e1 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_batch_col<T, N, PP, U, false>,     \
-                                                          N / PP * batch_tc<T>(N, PP), 0);         \
-        e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_batch_col<T, N, PP, U, true>,      \
-                                                          N / PP * batch_tc<T>(N, PP), 0);         \
-        break;
-        PFC_BATCH_CLASSES(PFC_LOCC)
-#undef PFC_LOCC
-    }
-    return lower_positive(a, b, e1, e2);
 }
 #endif
 

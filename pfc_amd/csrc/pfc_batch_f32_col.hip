@@ -3,6 +3,4 @@
 #include "pfc_batch.hpp"
 
 template void pfc::BatchLaunch<float>::col(int, bool, const pfc::BatchItem*, int,
-                                         const pfc::BatchSys<float>*, int,
-                                         hipStream_t);
-template int pfc::BatchLaunch<float>::col_occupancy(int);
+                                         const pfc::BatchSys<float>*, hipStream_t);

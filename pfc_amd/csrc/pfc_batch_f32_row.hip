@@ -3,6 +3,5 @@
 #include "pfc_batch.hpp"
 
 template void pfc::BatchLaunch<float>::row(int, bool, const pfc::BatchItem*, int,
-                                         const pfc::BatchSys<float>*, const int*, int, int,
+                                         const pfc::BatchSys<float>*, const int*, int,
                                          hipStream_t);
-template int pfc::BatchLaunch<float>::row_occupancy(int);

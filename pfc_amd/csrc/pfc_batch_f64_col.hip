@@ -3,6 +3,4 @@
 #include "pfc_batch.hpp"
 
 template void pfc::BatchLaunch<double>::col(int, bool, const pfc::BatchItem*, int,
-                                         const pfc::BatchSys<double>*, int,
-                                         hipStream_t);
-template int pfc::BatchLaunch<double>::col_occupancy(int);
+                                         const pfc::BatchSys<double>*, hipStream_t);

@@ -3,6 +3,5 @@
 #include "pfc_batch.hpp"
 
 template void pfc::BatchLaunch<double>::row(int, bool, const pfc::BatchItem*, int,
-                                         const pfc::BatchSys<double>*, const int*, int, int,
+                                         const pfc::BatchSys<double>*, const int*, int,
                                          hipStream_t);
-template int pfc::BatchLaunch<double>::row_occupancy(int);

@@ -1,5 +1,5 @@
 """Batch form (pfc_batch_*): many systems of uneven sizes, U = 1 and 3, stepped
-together by one persistent launch per pass, steps replayed from HIP graphs.
+together by one launch per chunk and pass, steps replayed from HIP graphs.
 Each system must follow the CPU oracle as a one-system run does (FP64) or
 stay within the mixed form's bound of the one-system engine (mixed); graph
 replays must equal eager launches bit for bit."""
