@@ -224,6 +224,9 @@ __device__ __forceinline__ void gstore_real(GBuf b, unsigned voff, unsigned soff
     }
 }
 
+#ifndef PFC_LDS_SWIZZLE
+#define PFC_LDS_SWIZZLE 1
+#endif
 constexpr int ilog(int b, int n) { return n < b ? 0 : 1 + ilog(b, n / b); }
 constexpr int ipow(int b, int e) { return e == 0 ? 1 : b * ipow(b, e - 1); }
 
@@ -239,6 +242,35 @@ struct BlockFFT {
                   "P must be 2, 4, 8 or 16 (L = 2^k) or 6, 12 (L = 3 * 2^k)");
     static constexpr int S = ilog(P, L);          // number of radix-P stages (L = 2^k)
     static constexpr int R0 = L / ipow(P, S);     // leading stage radix (1 if none)
+
+    // LDS index swizzle of the exchanges (L = 2^k).  LDS serves 256 bytes
+    // (64 banks) per cycle, i.e. 2^SWB consecutive indices of one transform
+    // when LS transforms interleave.  The Stockham writes of the early stages
+    // are strided (index j*R + r at Ns = 1), so consecutive lanes fall into a
+    // few banks; XORing the low SWB bits of the index with c ^ (c << 3),
+    // c = index >> SWB, spreads them (the gather reads, 2^SWB consecutive
+    // indices, stay a permutation of one bank row).  The map is linear over
+    // GF(2), so the swizzle of a lane's base index is computed once and each
+    // access XORs a compile-time constant (base and offset bits are disjoint).
+    // Modelled on every one-system shape: write conflicts from ~0.85 extra
+    // cycles per ideal cycle to ~0.1.  But each access then needs its own
+    // address register instead of an immediate offset, and measured at 4096^2
+    // with three fields that pays only where registers are to spare: FP64
+    // single-column transforms (column pass 447 -> 418 us).  FP32 and
+    // interleaved rows lost (mixed row pass 229 -> 309 us from spills, FP32
+    // column pass down to one block per CU, FP64 row pass +3.5 %), so the
+    // swizzle is on for FP64 with LS = 1 only.
+    static constexpr int LSB = (int)sizeof(cpx<T>) * LS;
+    static constexpr bool SWIZZLE = PFC_LDS_SWIZZLE && POW2 && sizeof(T) == 8 && LS == 1;
+    static constexpr int SWB = (!SWIZZLE || LSB >= 256) ? 0 : ilog(2, 256 / LSB);
+    __device__ static __forceinline__ int sw(int i) {
+        if constexpr (SWB == 0) {
+            return i;
+        } else {
+            const int c = i >> SWB;
+            return i ^ ((c ^ (c << 3)) & ((1 << SWB) - 1));
+        }
+    }
 
     // L = 3 * 2^k: stage s has radix gcd(P, L / (radices of stages < s)), so
     // the first stage takes the factor 3 (radix 12 or 6), the rest radix 4 or 2.
@@ -329,9 +361,10 @@ struct BlockFFT {
 #pragma unroll
                 for (int r = 0; r < R; r++) in[b + r * B] = x[r];
             } else {
-                const int base = (j - k) * R + k;
+                // base = (j - k) R + k and r Ns occupy disjoint bits
+                const int base = sw((j - k) * R + k);
 #pragma unroll
-                for (int r = 0; r < R; r++) lds[(base + r * Ns) * LS] = x[r];
+                for (int r = 0; r < R; r++) lds[(SWB ? (base ^ sw(r * Ns)) : (base + r * Ns)) * LS] = x[r];
             }
         }
     }
@@ -342,9 +375,11 @@ struct BlockFFT {
         constexpr int B = P / R;
 #pragma unroll
         for (int b = 0; b < B; b++) {
-            const int j = t + b * TPB;
+            // j < L / R: j and r L / R occupy disjoint bits
+            const int j = sw(t + b * TPB);
 #pragma unroll
-            for (int r = 0; r < R; r++) in[b + r * B] = lds[(j + r * (L / R)) * LS];
+            for (int r = 0; r < R; r++)
+                in[b + r * B] = lds[(SWB ? (j ^ sw(r * (L / R))) : (j + r * (L / R))) * LS];
         }
     }
 

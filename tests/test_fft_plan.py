@@ -1,0 +1,157 @@
+"""CPU emulation of the device FFT plan (BlockFFT in pfc_amd/csrc/fft_device.hpp):
+the stage radices, the Stockham write and gather indices, the LDS index
+swizzle and its composition from a per-lane base and per-access constants,
+for LS transforms interleaved in one buffer.  Each emulated transform must
+equal numpy's FFT, so a wrong stage index, a swizzle that is not a bijection
+or a base/offset pair that is not bit-disjoint fails here.  The device turns
+the swizzle on for FP64 single transforms (E = 16, LS = 1) only; the
+emulation also runs it on every other power-of-two shape, so the index
+algebra is checked wherever it could be turned on."""
+import math
+
+import numpy as np
+import pytest
+
+
+def ilog(b, n):
+    return 0 if n < b else 1 + ilog(b, n // b)
+
+
+class Plan:
+    """Index arithmetic of BlockFFT<T, L, TPB, DIR, TWS, LS> (E = sizeof(cpx<T>))."""
+
+    def __init__(self, L, TPB, LS, E, swizzle=None):
+        self.L, self.TPB, self.LS, self.E = L, TPB, LS, E
+        self.P = L // TPB
+        self.pow2 = L & (L - 1) == 0
+        if swizzle is None:   # the device's rule
+            swizzle = E == 16 and LS == 1
+        lsb = E * LS
+        self.SWB = 0 if (not swizzle or not self.pow2 or lsb >= 256) else ilog(2, 256 // lsb)
+
+    def sw(self, i):
+        if self.SWB == 0:
+            return i
+        c = i >> self.SWB
+        return i ^ ((c ^ (c << 3)) & ((1 << self.SWB) - 1))
+
+    def radices(self):
+        L, P = self.L, self.P
+        if self.pow2:
+            S = ilog(P, L)
+            R0 = L // P**S
+            return ([R0] if R0 > 1 else []) + [P] * S
+        out, rem = [], L
+        while rem > 1:
+            g = math.gcd(P, rem)
+            out.append(g)
+            rem //= g
+        return out
+
+    def run(self, x, DIR):
+        """x: (LS, L) complex; returns the LS transforms, DIR = -1 forward."""
+        L, TPB, P, LS = self.L, self.TPB, self.P, self.LS
+        t = np.arange(TPB)
+        # thread t holds elements t + q TPB (natural order)
+        v = np.stack([x[:, t + q * TPB] for q in range(P)], axis=2)   # (LS, TPB, P)
+        lds = np.full(L * LS, np.nan, dtype=complex)
+        rr = np.arange(LS)[:, None]
+        rads = self.radices()
+        Ns = 1
+        for s, R in enumerate(rads):
+            B = P // R
+            last = s == len(rads) - 1
+            nv = v.copy()
+            for b in range(B):
+                j = t + b * TPB
+                k = j & (Ns - 1) if self.pow2 else j % Ns
+                xs = [v[:, :, b + r * B] for r in range(R)]
+                if Ns > 1:
+                    step = L // (Ns * R)
+                    xs = [xs[r] * np.exp(DIR * 2j * np.pi * (k * r * step) / L) for r in range(R)]
+                X = [sum(xs[r] * np.exp(DIR * 2j * np.pi * r * kk / R) for r in range(R))
+                     for kk in range(R)]
+                if last:
+                    for r in range(R):
+                        nv[:, :, b + r * B] = X[r]
+                else:
+                    logical = (j - k) * R + k
+                    base = np.array([self.sw(int(a)) for a in logical])
+                    for r in range(R):
+                        if self.SWB:
+                            assert not np.any(logical & (r * Ns))
+                            idx = base ^ self.sw(r * Ns)
+                        else:
+                            idx = base + r * Ns
+                        lds[idx[None, :] * LS + rr] = X[r]
+            if last:
+                v = nv
+                break
+            Ns *= R
+            Rn = rads[s + 1]
+            Bn = P // Rn
+            for b in range(Bn):
+                j = t + b * TPB
+                jb = np.array([self.sw(int(a)) for a in j])
+                for r in range(Rn):
+                    if self.SWB:
+                        assert not np.any(j & (r * (L // Rn)))
+                        idx = jb ^ self.sw(r * (L // Rn))
+                    else:
+                        idx = jb + r * (L // Rn)
+                    v[:, :, b + r * Bn] = lds[idx[None, :] * LS + rr]
+        out = np.empty_like(x)
+        for q in range(P):
+            out[:, t + q * TPB] = v[:, :, q]
+        return out
+
+
+def check(L, TPB, LS, E, rng, swizzle=None):
+    p = Plan(L, TPB, LS, E, swizzle)
+    x = rng.standard_normal((LS, L)) + 1j * rng.standard_normal((LS, L))
+    ref = np.fft.fft(x, axis=1)
+    scale = np.abs(ref).max()
+    assert np.abs(p.run(x, -1) - ref).max() < 1e-10 * scale
+    assert np.abs(p.run(x, +1) - np.fft.ifft(x, axis=1) * L).max() < 1e-10 * scale
+    return p
+
+
+# (L, TPB, LS, E): the row transforms (L = Nx/2, LS = rows per block), the
+# column transforms (L = Ny, LS = columns per block) and the batch shapes
+SHAPES = [
+    (2048, 256, 4, 8), (2048, 256, 2, 16), (4096, 512, 1, 8), (4096, 512, 1, 16),
+    (1024, 128, 8, 8), (1024, 128, 4, 16), (2048, 256, 1, 8), (2048, 256, 1, 16),
+    (4096, 1024, 2, 8), (4096, 512, 1, 16), (8192, 1024, 1, 8), (8192, 1024, 1, 16),
+    (512, 64, 4, 8), (512, 64, 2, 16), (1024, 128, 2, 8), (1024, 128, 2, 16),
+    (256, 32, 8, 8), (256, 32, 8, 16), (128, 16, 16, 8), (128, 16, 16, 16),
+    (64, 8, 32, 8), (256, 16, 16, 8), (512, 32, 8, 16), (1024, 64, 4, 8),
+    (2048, 128, 8, 8), (2048, 128, 4, 16), (1024, 256, 4, 8), (2048, 512, 2, 8),
+    (128, 64, 16, 8), (64, 16, 32, 16),
+]
+
+
+@pytest.mark.parametrize("swizzle", [None, True])
+@pytest.mark.parametrize("L,TPB,LS,E", SHAPES)
+def test_power_of_two_plans(L, TPB, LS, E, swizzle):
+    p = check(L, TPB, LS, E, np.random.default_rng(L + TPB + LS + E), swizzle)
+    on = swizzle or (E == 16 and LS == 1)
+    assert p.SWB == (0 if (not on or E * LS >= 256) else ilog(2, 256 // (E * LS)))
+
+
+@pytest.mark.parametrize("L", [12, 24, 48, 96, 192, 384, 768, 1536])
+@pytest.mark.parametrize("P", [6, 12])
+def test_three_times_power_of_two_plans(L, P):
+    if L % P:
+        pytest.skip("P does not divide L")
+    p = check(L, L // P, 1, 16, np.random.default_rng(L * P), swizzle=True)
+    assert p.SWB == 0   # never for 3 * 2^k
+    assert p.radices()[0] == P and all(r in (2, 4) for r in p.radices()[1:])
+
+
+def test_swizzle_is_a_bijection_within_bank_rows():
+    for E, LS in ((8, 1), (16, 1), (8, 4), (16, 2), (8, 2)):
+        p = Plan(4096, 512, LS, E, swizzle=True)
+        idx = np.arange(4096)
+        s = np.array([p.sw(int(i)) for i in idx])
+        assert sorted(s) == list(idx)
+        assert np.all(s >> p.SWB == idx >> p.SWB)   # only the low bits move
