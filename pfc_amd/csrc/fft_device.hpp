@@ -230,6 +230,14 @@ __device__ __forceinline__ void gstore_real(GBuf b, unsigned voff, unsigned soff
 constexpr int ilog(int b, int n) { return n < b ? 0 : 1 + ilog(b, n / b); }
 constexpr int ipow(int b, int e) { return e == 0 ? 1 : b * ipow(b, e - 1); }
 
+// Compact per-stage twiddle table in LDS (BlockFFT::kTab entries, filled
+// from tab_src), read by stage-local index so that a wave's reads are
+// contiguous or broadcast.
+template <typename T>
+struct StageTab {
+    const cpx<T>* p;
+};
+
 // One length-L transform over TPB cooperating threads (thread index t).
 // lds must hold L elements at lds[idx * LS] (LS = element stride, lets
 // several interleaved transforms share a buffer without bank conflicts).
@@ -309,6 +317,56 @@ struct BlockFFT {
         return w;
     }
 
+    // Compact stage table (L = 2^k): for each stage with Ns > 1, in execution
+    // order, Ns entries w1[k] = e^{-2 pi i k step/L} (step = L/(Ns R)) and,
+    // in FP32, Ns entries wh[k] = e^{-2 pi i k (R/2) step/L}.
+    // The stages with Ns > 1 have Ns = F, F P, F P^2, ... up to L / P, F the
+    // first (R0, or P without a leading stage), so the blocks before the one
+    // of Ns sum to NW (Ns - F) / (P - 1) entries.
+    static constexpr int NW = (sizeof(T) == 8 && PFC_TW_RECURRENCE) ? 1 : 2;
+    static constexpr int F = R0 > 1 ? R0 : P;
+    __device__ static __forceinline__ int soff(int Ns) { return NW * (Ns - F) / (P - 1); }
+    static constexpr int kTab = POW2 ? NW * (L - F) / (P - 1) : 0;
+    // twiddle exponent m (e^{-2 pi i m/L}) of compact entry i (the stage of
+    // Ns has radix R0 if Ns = 1 and a leading stage exists, else P, so every
+    // stage with Ns > 1 has radix P and step L / (Ns P))
+    // (selects over the compile-time stages, no branches)
+    static constexpr int NSTG = POW2 ? ilog(P, L / F) : 0;
+    __device__ static __forceinline__ int tab_src(int i) {
+        int m = 0;
+#pragma unroll
+        for (int s = 0, ns = F; s < NSTG; s++, ns *= P) {
+            const int step = L / (ns * P);
+            m = (i >= 0 && i < ns) ? i * step : m;
+            i -= ns;
+            if constexpr (NW == 2) {
+                m = (i >= 0 && i < ns) ? i * (P / 2) * step : m;
+                i -= ns;
+            }
+        }
+        return m;
+    }
+    template <typename TW>
+    __device__ static __forceinline__ cpx<T> tw_w1(TW tw, int k, int, int step, unsigned z) {
+        return twid(tw, k * step, z);
+    }
+    template <typename TW>
+    __device__ static __forceinline__ cpx<T> tw_wh(TW tw, int k, int, int step, int Hh, unsigned z) {
+        return twid(tw, k * Hh * step, z);
+    }
+    // z: the transform's opaque zero, so that the compiler re-reads the table
+    // per transform instead of keeping every stage's twiddles live
+    __device__ static __forceinline__ cpx<T> tw_w1(StageTab<T> st, int k, int Ns, int, unsigned z) {
+        cpx<T> w = st.p[soff(Ns) + k + (int)z];
+        if constexpr (DIR > 0) w.y = -w.y;
+        return w;
+    }
+    __device__ static __forceinline__ cpx<T> tw_wh(StageTab<T> st, int k, int Ns, int, int, unsigned z) {
+        cpx<T> w = st.p[soff(Ns) + Ns + k + (int)z];
+        if constexpr (DIR > 0) w.y = -w.y;
+        return w;
+    }
+
     // One radix-R Stockham stage on butterflies j = t + b*TPB, inputs in in[].
     template <int R, bool LAST, typename TW>
     __device__ static __forceinline__ void stage(cpx<T> (&in)[P], int Ns, cpx<T>* lds, TW tw,
@@ -328,7 +386,7 @@ struct BlockFFT {
                     // FP64: powers w^r of one table twiddle by recurrence
                     // (error grows by ~r ulp, r < 16), far fewer loads and
                     // live registers than one load per r
-                    const cpx<T> w1 = twid(tw, k * step, z);
+                    const cpx<T> w1 = tw_w1(tw, k, Ns, step, z);
                     cpx<T> w = w1;
 #pragma unroll
                     for (int r = 1; r < R; r++) {
@@ -339,8 +397,8 @@ struct BlockFFT {
                     // FP32: two table twiddles w and w^(R/2); every other
                     // power is at most two products away (~2 ulp)
                     constexpr int Hh = R / 2;
-                    const cpx<T> w1 = twid(tw, k * step, z);
-                    const cpx<T> wh = twid(tw, k * Hh * step, z);
+                    const cpx<T> w1 = tw_w1(tw, k, Ns, step, z);
+                    const cpx<T> wh = tw_wh(tw, k, Ns, step, Hh, z);
                     cpx<T> lo[Hh];
                     lo[0] = {(T)1, (T)0};
                     lo[1] = w1;
@@ -351,6 +409,8 @@ struct BlockFFT {
                         const cpx<T> w = (r < Hh) ? lo[r] : (r == Hh ? wh : cmul(wh, lo[r - Hh]));
                         x[r] = cmul(x[r], w);
                     }
+                } else if constexpr (R == 2) {
+                    x[1] = cmul(x[1], tw_w1(tw, k, Ns, step, z));
                 } else {
 #pragma unroll
                     for (int r = 1; r < R; r++) x[r] = cmul(x[r], twid(tw, k * r * step, z));

@@ -84,12 +84,13 @@ __global__ __launch_bounds__(N / 2 / PP * batch_tr<T>(N, PP)) PFC_WAVES(row_wave
     const int* __restrict__ tdev, int toff) {
     constexpr int TPB = N / 2 / PP, TR = batch_tr<T>(N, PP);
     __shared__ cpx<T> lds[N / 2 * TR];
+    __shared__ __attribute__((aligned(16))) cpx<T> ltab[row_tab_size<T, N, TPB, TR>()];
     const int tstep = *tdev + toff;   // the step's index (NaN reports)
     if ((int)blockIdx.x >= nitems) return;
     const BatchItem it = items[blockIdx.x];
     const BatchSys<T>& S = sys[it.sys];
-    row_step_body<T, N, TPB, TR, U, REF>(lds, it.blk, S.I, S.J, S.K, S.fstride, S.sI, S.sJ, S.twx,
-                                         S.prm, S.nan, tstep);
+    row_step_body<T, N, TPB, TR, U, REF>(lds, ltab, it.blk, S.I, S.J, S.K, S.fstride, S.sI, S.sJ,
+                                         S.twx, S.prm, S.nan, tstep);
 }
 
 // Column pass of one class: block w runs item w.

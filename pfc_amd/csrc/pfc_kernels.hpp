@@ -317,55 +317,70 @@ __device__ __forceinline__ cpx<T> row_wt(GBuf twx, int t) {
     return gload<T>(twx, (unsigned)t * sizeof(cpx<T>), z);
 }
 
-template <typename T, int H, int TPB, int LS>
-__device__ __forceinline__ void c2r_pre(cpx<T> (&v)[H / TPB], cpx<T>* lds, cpx<T> wt, int t) {
+// Index H - k of the Hermitian partner of k = t + q TPB (0 < k < H), and 0
+// for k = 0 (q = t = 0), where the caller selects the packed DC/Nyquist value
+// instead.  For q >= 1, k >= TPB > 0.
+template <int H, int TPB>
+__device__ __forceinline__ int row_partner(int t, int q) {
+    return q == 0 ? (t == 0 ? 0 : H - t) : H - t - q * TPB;
+}
+
+// wt is either the twiddle or a pointer to it in LDS, read after the barrier
+// (so that a table written before the call is complete).
+template <typename T>
+__device__ __forceinline__ cpx<T> wt_value(cpx<T> w) { return w; }
+template <typename T>
+__device__ __forceinline__ cpx<T> wt_value(const cpx<T>* w) {
+    int z = 0;   // opaque: re-read per call rather than kept live between transforms
+    asm volatile("" : "+s"(z));
+    return w[z];
+}
+
+template <typename T, int H, int TPB, int LS, typename WT>
+__device__ __forceinline__ void c2r_pre(cpx<T> (&v)[H / TPB], cpx<T>* lds, WT wtv, int t) {
     constexpr int P = H / TPB;
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < P; q++) lds[(t + q * TPB) * LS] = v[q];
     __syncthreads();
+    const cpx<T> wt = wt_value<T>(wtv);
 #pragma unroll
     for (int q = 0; q < P; q++) {
-        const int k = t + q * TPB;
-        if (k == 0) {
-            const T x0 = v[q].x, xh = v[q].y;   // packed DC and Nyquist
-            v[q] = {x0 + xh, x0 - xh};
-        } else {
-            const cpx<T> a = v[q];
-            const cpx<T> b = cconj(lds[(H - k) * LS]);
-            const cpx<T> xe = cadd(a, b);
-            const cpx<T> w = cmul(wt, root<T>(q, 2 * P, -1));   // e^{-2 pi i k/Nx}
-            const cpx<T> xo = cmulc(csub(a, b), w);   // * e^{+2 pi i k/Nx}
-            v[q] = {xe.x - xo.y, xe.y + xo.x};              // xe + i xo
-        }
+        // k = t + q TPB is 0 only for q = t = 0 (packed DC and Nyquist): the
+        // general value is computed for every lane (partner index 0 there)
+        // and replaced by a select, so there is no branch
+        const cpx<T> a = v[q];
+        const cpx<T> b = cconj(lds[row_partner<H, TPB>(t, q) * LS]);
+        const cpx<T> xe = cadd(a, b);
+        const cpx<T> w = cmul(wt, root<T>(q, 2 * P, -1));   // e^{-2 pi i k/Nx}
+        const cpx<T> xo = cmulc(csub(a, b), w);   // * e^{+2 pi i k/Nx}
+        v[q] = {xe.x - xo.y, xe.y + xo.x};              // xe + i xo
+        if (q == 0 && t == 0) v[q] = {a.x + a.y, a.x - a.y};
     }
 }
 
 // R2C post-processing: v holds Z = FFT_H(z), z[n] = x[2n] + i x[2n+1], in
 // natural order; produces X[k] (k < H) with X[0] packed as (DC, Nyquist).
-template <typename T, int H, int TPB, int LS>
-__device__ __forceinline__ void r2c_post(cpx<T> (&v)[H / TPB], cpx<T>* lds, cpx<T> wt, int t) {
+template <typename T, int H, int TPB, int LS, typename WT>
+__device__ __forceinline__ void r2c_post(cpx<T> (&v)[H / TPB], cpx<T>* lds, WT wtv, int t) {
     constexpr int P = H / TPB;
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < P; q++) lds[(t + q * TPB) * LS] = v[q];
     __syncthreads();
+    const cpx<T> wt = wt_value<T>(wtv);
 #pragma unroll
     for (int q = 0; q < P; q++) {
-        const int k = t + q * TPB;
-        if (k == 0) {
-            const T re = v[q].x, im = v[q].y;
-            v[q] = {re + im, re - im};   // packed (DC, Nyquist)
-        } else {
-            const cpx<T> a = v[q];
-            const cpx<T> b = cconj(lds[(H - k) * LS]);
-            const cpx<T> s = cadd(a, b);   // 2 Xe
-            const cpx<T> d = csub(a, b);   // 2i Xo
-            const cpx<T> xo2 = {d.y, -d.x};   // 2 Xo
-            const cpx<T> w = cmul(wt, root<T>(q, 2 * P, -1));   // e^{-2 pi i k/Nx}
-            const cpx<T> r = cadd(s, cmul(xo2, w));
-            v[q] = cscale(r, (T)0.5);
-        }
+        // k = 0 only for q = t = 0; selected as in c2r_pre
+        const cpx<T> a = v[q];
+        const cpx<T> b = cconj(lds[row_partner<H, TPB>(t, q) * LS]);
+        const cpx<T> s = cadd(a, b);   // 2 Xe
+        const cpx<T> d = csub(a, b);   // 2i Xo
+        const cpx<T> xo2 = {d.y, -d.x};   // 2 Xo
+        const cpx<T> w = cmul(wt, root<T>(q, 2 * P, -1));   // e^{-2 pi i k/Nx}
+        const cpx<T> r = cadd(s, cmul(xo2, w));
+        v[q] = cscale(r, (T)0.5);
+        if (q == 0 && t == 0) v[q] = {a.x + a.y, a.x - a.y};   // packed (DC, Nyquist)
     }
 }
 
@@ -423,19 +438,47 @@ __device__ __forceinline__ void nonlinear(const DevParams* __restrict__ p, T (&m
 // row index fastest, so the TR rows' elements of one kx are adjacent lanes
 // and adjacent addresses in the column-major buffers (TR*sizeof(E) bytes
 // contiguous per kx).
+// Entries of the row pass's LDS twiddle table for L = 2^k row transforms:
+// the transform's compact stage table, then e^{-2 pi i t/Nx} for t < TPB
+// (the real-FFT pre/post twiddle), padded to a multiple of the block's
+// TPB*TR lanes so that every lane gathers and writes without a branch;
+// 1 (unused) for 3 * 2^k.
+template <typename T, int NX, int TPB>
+constexpr int row_tab_used() {
+    using F = BlockFFT<T, NX / 2, TPB, -1, 2, 1>;
+    return F::POW2 ? F::kTab + TPB : 0;
+}
+template <typename T, int NX, int TPB, int TR>
+constexpr int row_tab_size() {
+    constexpr int n = row_tab_used<T, NX, TPB>(), nt = TPB * TR;
+    return n ? (n + nt - 1) / nt * nt : 1;
+}
+
 // Body of the row pass for row group g (rows g*TR ... g*TR + TR - 1) on a
-// block of TPB*TR lanes; lds holds H*TR elements.  Shared by the one-system
-// kernel below and the batch kernels.
+// block of TPB*TR lanes; lds holds H*TR elements, ltab row_tab_size()
+// (16-byte aligned).  Shared by the one-system kernel below and the batch
+// kernels.
+//
+// For L = 2^k the transforms take their twiddles from ltab, which each block
+// gathers from the Nx-point table before it loads the fields: global loads
+// and stores complete in issue order under one counter, so twiddle loads
+// issued during the transforms would make the C2R of field 0 wait for every
+// field's loads and the R2C of field u + 1 wait until field u's stores have
+// drained.
 template <typename T, int NX, int TPB, int TR, int U, bool REFRESH>
 __device__ __forceinline__ void row_step_body(
-    cpx<T>* lds, int g, const cpx<T>* __restrict__ I, cpx<T>* __restrict__ J,
+    cpx<T>* lds, cpx<T>* ltab, int g, const cpx<T>* __restrict__ I, cpx<T>* __restrict__ J,
     cpx<T>* __restrict__ K, size_t fstride, int sI, int sJ, const cpx<T>* __restrict__ twx_,
     const DevParams* __restrict__ prm, int* __restrict__ nan_step, int tstep) {
     constexpr int H = NX / 2;
     constexpr int P = H / TPB;
+    constexpr int NT = TPB * TR;
     constexpr unsigned E = sizeof(cpx<T>);
     using FFTF = BlockFFT<T, H, TPB, -1, 2, TR>;
     using FFTI = BlockFFT<T, H, TPB, +1, 2, TR>;
+    constexpr bool STAB = FFTF::POW2;
+    constexpr int NTAB = row_tab_used<T, NX, TPB>();
+    constexpr int NI = row_tab_size<T, NX, TPB, TR>() / NT;
     const int tid = threadIdx.x;
     const int r = tid % TR;
     const int t = tid / TR;
@@ -446,6 +489,18 @@ __device__ __forceinline__ void row_step_body(
     const unsigned voff = lay_row_off(sJ, H, t, y, E), qstep = lay_row_step(sJ, TPB, E);
 
     PFC_PROBE_AT(0, 0);
+    cpx<T> tv[NI > 0 ? NI : 1];
+    if constexpr (STAB) {
+#pragma unroll
+        for (int i = 0; i < NI; i++) {
+            // unconditional (index clamped), so that no branch separates the
+            // gather from the field loads
+            const int e = tid + i * NT;
+            const int m = e < FFTF::kTab ? 2 * FFTF::tab_src(e) : (e < NTAB ? e - FFTF::kTab : 0);
+            tv[i] = gload<T>(twx, (unsigned)m * E, 0u);
+        }
+        __builtin_amdgcn_sched_barrier(0);   // issued before the field loads
+    }
     cpx<T> m[U][P];
 #pragma unroll
     for (int u = 0; u < U; u++) {
@@ -453,10 +508,35 @@ __device__ __forceinline__ void row_step_body(
 #pragma unroll
         for (int q = 0; q < P; q++) m[u][q] = gload<T>(src, ioff, q * istep);
     }
+    if constexpr (STAB) {
+        // keep the table's LDS writes (which wait for its gather) after the
+        // field loads have been issued
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int i = 0; i < NI; i++) ltab[tid + i * NT] = tv[i];
+    }
+    // the transforms' twiddles and the pre/post twiddle of lane t (ltab is
+    // read only after the first barrier inside c2r_pre)
+    const StageTab<T> stab{ltab};
+    const cpx<T>* wtp = ltab + FFTF::kTab + t;
+    auto fwd = [&](cpx<T>(&v)[P]) {
+        if constexpr (STAB) {
+            FFTF::run(v, L, stab, t);
+            r2c_post<T, H, TPB, TR>(v, L, wtp, t);
+        } else {
+            FFTF::run(v, L, twx, t);
+            r2c_post<T, H, TPB, TR>(v, L, row_wt<T>(twx, t), t);
+        }
+    };
 #pragma unroll
     for (int u = 0; u < U; u++) {
-        c2r_pre<T, H, TPB, TR>(m[u], L, row_wt<T>(twx, t), t);
-        FFTI::run(m[u], L, twx, t);
+        if constexpr (STAB) {
+            c2r_pre<T, H, TPB, TR>(m[u], L, wtp, t);
+            FFTI::run(m[u], L, stab, t);
+        } else {
+            c2r_pre<T, H, TPB, TR>(m[u], L, row_wt<T>(twx, t), t);
+            FFTI::run(m[u], L, twx, t);
+        }
         if (u == 0) PFC_PROBE_AT(0, 1);
     }
     PFC_PROBE_AT(0, 2);
@@ -472,8 +552,7 @@ __device__ __forceinline__ void row_step_body(
             cpx<T> w[P];
 #pragma unroll
             for (int q = 0; q < P; q++) w[q] = m[u][q];
-            FFTF::run(w, L, twx, t);
-            r2c_post<T, H, TPB, TR>(w, L, row_wt<T>(twx, t), t);
+            fwd(w);
             const GBuf dst = gbuf(K + u * fstride);
 #pragma unroll
             for (int q = 0; q < P; q++) gstore<T>(dst, voff, q * qstep, w[q]);
@@ -495,8 +574,7 @@ __device__ __forceinline__ void row_step_body(
     PFC_PROBE_AT(0, 3);
 #pragma unroll
     for (int u = 0; u < U; u++) {
-        FFTF::run(m[u], L, twx, t);
-        r2c_post<T, H, TPB, TR>(m[u], L, row_wt<T>(twx, t), t);
+        fwd(m[u]);
         const GBuf dst = gbuf(J + u * fstride);
 #pragma unroll
         for (int q = 0; q < P; q++) gstore<T, PFC_ROWJ_AUX>(dst, voff, q * qstep, m[u][q]);
@@ -510,11 +588,12 @@ __global__ __launch_bounds__(TPB* TR) PFC_WAVES(row_waves<T>()) void k_row_step(
     size_t fstride, int sI, int sJ, const cpx<T>* __restrict__ twx_,
     const DevParams* __restrict__ prm, int* __restrict__ nan_step, int tstep) {
     __shared__ cpx<T> lds[NX / 2 * TR];
+    __shared__ __attribute__((aligned(16))) cpx<T> ltab[row_tab_size<T, NX, TPB, TR>()];
 #ifndef PFC_ROW_XCD
 #define PFC_ROW_XCD 1
 #endif
     const int g = PFC_ROW_XCD ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
-    row_step_body<T, NX, TPB, TR, U, REFRESH>(lds, g, I, J, K, fstride, sI, sJ, twx_, prm,
+    row_step_body<T, NX, TPB, TR, U, REFRESH>(lds, ltab, g, I, J, K, fstride, sI, sJ, twx_, prm,
                                               nan_step, tstep);
 }
 

@@ -155,3 +155,69 @@ def test_swizzle_is_a_bijection_within_bank_rows():
         s = np.array([p.sw(int(i)) for i in idx])
         assert sorted(s) == list(idx)
         assert np.all(s >> p.SWB == idx >> p.SWB)   # only the low bits move
+
+
+def stage_table(L, P, NW):
+    """The compact stage table as the device lays it out (BlockFFT::soff,
+    kTab, tab_src): per stage with Ns > 1, Ns entries w1 and, with NW = 2,
+    Ns entries wh; F is the first such Ns."""
+    S = ilog(P, L)
+    R0 = L // P**S
+    F = R0 if R0 > 1 else P
+    soff = lambda Ns: NW * (Ns - F) // (P - 1)
+    kTab = NW * (L - F) // (P - 1)
+    nstg = ilog(P, L // F)
+
+    def tab_src(i):
+        m, ns = 0, F
+        for _ in range(nstg):
+            step = L // (ns * P)
+            m = i * step if 0 <= i < ns else m
+            i -= ns
+            if NW == 2:
+                m = i * (P // 2) * step if 0 <= i < ns else m
+                i -= ns
+            ns *= P
+        return m
+    return soff, kTab, np.exp(-2j * np.pi * np.array([tab_src(i) for i in range(kTab)]) / L)
+
+
+@pytest.mark.parametrize("NW", [1, 2])
+@pytest.mark.parametrize("L,TPB", [(2048, 256), (2048, 128), (1024, 128), (1024, 64), (512, 64),
+                                   (512, 32), (256, 32), (256, 16), (128, 16), (64, 8), (32, 8),
+                                   (4096, 512), (8192, 1024), (4096, 256)])
+def test_compact_stage_table(L, TPB, NW):
+    """Every stage that multiplies twiddles finds w1[k] = e^{-2 pi i k step/L}
+    (and wh[k] = w1[k]^(R/2)) at its block of the table, blocks do not
+    overlap, and the table has exactly kTab entries."""
+    p = Plan(L, TPB, 1, 8 * NW)
+    soff, kTab, tab = stage_table(L, p.P, NW)
+    Ns, used = 1, set()
+    for R in p.radices():
+        if Ns > 1:
+            assert R == p.P
+            step = L // (Ns * R)
+            k = np.arange(Ns)
+            o = soff(Ns)
+            assert np.allclose(tab[o + k], np.exp(-2j * np.pi * k * step / L), atol=1e-15)
+            idx = set(range(o, o + NW * Ns))
+            if NW == 2:
+                assert np.allclose(tab[o + Ns + k], np.exp(-2j * np.pi * k * (R // 2) * step / L),
+                                   atol=1e-15)
+            assert not (idx & used)
+            used |= idx
+        Ns *= R
+    assert used == set(range(kTab))
+
+
+@pytest.mark.parametrize("H,TPB", [(2048, 256), (2048, 128), (1024, 64), (192, 16), (384, 32),
+                                   (768, 64), (96, 8), (32, 8), (4096, 512)])
+def test_row_partner_index(H, TPB):
+    """row_partner (pfc_kernels.hpp): the Hermitian partner H - k of
+    k = t + q TPB, 0 for k = 0, for power-of-two and 3 * 2^k half lengths."""
+    def row_partner(t, q):
+        return (0 if t == 0 else H - t) if q == 0 else H - t - q * TPB
+    for q in range(H // TPB):
+        for t in range(TPB):
+            k = t + q * TPB
+            assert row_partner(t, q) == (H - k) % H

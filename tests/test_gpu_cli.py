@@ -89,8 +89,13 @@ def test_graphene_run_matches_oracle(gpu, tmp_path, c, topt):
     o.set_fields(f[None])
     olog, snaps = o.relax(250, 100, Topt=topt, xopt=1, yopt=1, snap_steps=(0, 100, 250))
     assert np.array_equal(log[:, 0], olog[:, 0])
-    np.testing.assert_allclose(log[:, 1:3], olog[:, 1:3], rtol=1e-11, atol=0)   # dx, dy
-    np.testing.assert_allclose(log[:, 3], olog[:, 3], rtol=1e-11, atol=1e-14)   # fave
+    # dx, dy: with topt = 50 the box comes out of five chained searches, each
+    # moving it by ~1e-12 relative per unit of energy round-off (see below),
+    # and the GPU's FMA contraction rounds differently from the oracle's
+    # unfused arithmetic (measured up to 1.9e-11)
+    np.testing.assert_allclose(log[:, 1:3], olog[:, 1:3], rtol=5e-11, atol=0)
+    # fave depends on the box and follows it; nave does not
+    np.testing.assert_allclose(log[:, 3], olog[:, 3], rtol=5e-11, atol=1e-14)   # fave
     np.testing.assert_allclose(log[:, 4], olog[:, 4], rtol=1e-11, atol=1e-14)   # nave
     # Each box search fits a parabola through energies whose second difference
     # is ~1e-9 of the energy, so round-off-level energy differences move the
