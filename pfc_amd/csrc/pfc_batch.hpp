@@ -98,9 +98,9 @@ template <typename T, int N, int PP, int U, bool REF>
 __global__ __launch_bounds__(N / PP * batch_tc<T>(N, PP)) PFC_WAVES(col_waves<T>()) void k_batch_col(
     const BatchItem* __restrict__ items, int nitems, const BatchSys<T>* __restrict__ sys) {
     constexpr int TPB = N / PP, TC = batch_tc<T>(N, PP);
-    constexpr bool LTAB = col_ltab<T, N, TC>();
+    constexpr bool LTAB = col_ltab<T, N, TPB, TC>();
     __shared__ cpx<T> lds[N * TC];
-    __shared__ __attribute__((aligned(16))) cpx<T> ltw[LTAB ? N : 1];
+    __shared__ __attribute__((aligned(16))) cpx<T> ltw[col_tw_entries<T, N, TPB, TC, LTAB>()];
     __shared__ __attribute__((aligned(16))) T lgy[LTAB ? N : 1];
     if ((int)blockIdx.x >= nitems) return;
     const BatchItem it = items[blockIdx.x];

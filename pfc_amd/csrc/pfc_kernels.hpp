@@ -653,6 +653,18 @@ __device__ __forceinline__ void hermitian_cols(cpx<T> (&M)[U][NY / TPB], cpx<T>*
     }
 }
 
+// Entries of the column pass's LDS twiddle table: for L = 2^k the compact
+// stage table (BlockFFT::kTab, read by stage-local index, conflict-free),
+// padded to a multiple of the block's TPB*TC lanes so that every lane
+// gathers and writes without a branch; for 3 * 2^k the full Ny-point table.
+template <typename T, int NY, int TPB, int TC, bool LTAB>
+constexpr int col_tw_entries() {
+    using F = BlockFFT<T, NY, TPB, -1, 1, 1>;
+    if (!LTAB) return 1;
+    if (!F::POW2) return NY;
+    return (F::kTab + TPB * TC - 1) / (TPB * TC) * (TPB * TC);
+}
+
 // Column pass of a step: TC columns per block (lane = column-major within the
 // block), each column all Ny rows; the extra last block runs kx = H.
 // fstride: row-spectral field stride (H*Ny), nstride: state stride ((H+1)*Ny).
@@ -695,11 +707,24 @@ __device__ __forceinline__ void col_step_body(
         const unsigned vi = lay_col_off(sI, H, jb.mode ? 0 : kx, t, E);
         const unsigned istep = lay_col_step(sI, H, TPB, E);
 
-        // tables first, then all fields' J
+        // tables first, then all fields' J.  For L = 2^k the twiddles are
+        // the compact stage table, gathered into registers ahead of the field
+        // loads and written to LDS after them (so that the write waits for
+        // the gather only); for 3 * 2^k the full table is copied.
+        constexpr bool CT = LTAB && FFTF::POW2;
+        constexpr int NT = TPB * TC;
+        constexpr int NI = CT ? col_tw_entries<T, NY, TPB, TC, LTAB>() / NT : 1;
+        cpx<T> tv[NI];
         if constexpr (LTAB) {
-            lds_copy16<NY * (int)E, TPB * TC>(twy, 0u, ltw, tid);
-            lds_copy16<NY * (int)sizeof(T), TPB * TC>(ktab, (unsigned)ktab_gy(H) * sizeof(T), lgy,
-                                                       tid);
+            if constexpr (!CT) lds_copy16<NY * (int)E, NT>(twy, 0u, ltw, tid);
+            lds_copy16<NY * (int)sizeof(T), NT>(ktab, (unsigned)ktab_gy(H) * sizeof(T), lgy, tid);
+        }
+        if constexpr (CT) {
+#pragma unroll
+            for (int i = 0; i < NI; i++) {
+                const int e = tid + i * NT;
+                tv[i] = gload<T>(twy, (unsigned)(e < FFTF::kTab ? FFTF::tab_src(e) : 0) * E, 0u);
+            }
         }
         const T gx = gload_real<T>(ktab, (unsigned)kx * sizeof(T));
         __builtin_amdgcn_sched_barrier(0);
@@ -710,13 +735,20 @@ __device__ __forceinline__ void col_step_body(
 #pragma unroll
             for (int q = 0; q < P; q++) M[u][q] = col_in<T>(src, vr, q * rstep, jb.mode);
         }
+        if constexpr (CT) {
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int i = 0; i < NI; i++) ltw[tid + i * NT] = tv[i];
+        }
         __syncthreads();   // the table copies are complete once the loads before them are
         auto fwd = [&](cpx<T>(&v)[P]) {
-            if constexpr (LTAB) FFTF::run(v, L, (const cpx<T>*)ltw, t);
+            if constexpr (CT) FFTF::run(v, L, StageTab<T>{ltw}, t);
+            else if constexpr (LTAB) FFTF::run(v, L, (const cpx<T>*)ltw, t);
             else FFTF::run(v, L, twy, t);
         };
         auto inv = [&](cpx<T>(&v)[P]) {
-            if constexpr (LTAB) FFTI::run(v, L, (const cpx<T>*)ltw, t);
+            if constexpr (CT) FFTI::run(v, L, StageTab<T>{ltw}, t);
+            else if constexpr (LTAB) FFTI::run(v, L, (const cpx<T>*)ltw, t);
             else FFTI::run(v, L, twy, t);
         };
 #pragma unroll
@@ -838,9 +870,10 @@ __device__ __forceinline__ void col_step_body(
 
 // twiddle and ky-factor tables in LDS where they fit beside the exchange
 // buffer (all sizes but FP64 Ny = 8192)
-template <typename T, int NY, int TC>
+template <typename T, int NY, int TPB, int TC>
 constexpr bool col_ltab() {
-    return NY * TC * sizeof(cpx<T>) + NY * sizeof(cpx<T>) + NY * sizeof(T) <= 160 * 1024;
+    return NY * TC * sizeof(cpx<T>) + col_tw_entries<T, NY, TPB, TC, true>() * sizeof(cpx<T>) +
+               NY * sizeof(T) <= 160 * 1024;
 }
 
 template <typename T, int NY, int TPB, int TC, int U, bool REFRESH>
@@ -849,9 +882,9 @@ __global__ __launch_bounds__(TPB* TC) PFC_WAVES(col_waves<T>()) void k_col_step(
     cpx<T>* __restrict__ I, size_t fstride, size_t nstride, int H, int sI, int sJ,
     const cpx<T>* __restrict__ twy_, const T* __restrict__ ktab_,
     const DevParams* __restrict__ prm, double* __restrict__ n0) {
-    constexpr bool LTAB = col_ltab<T, NY, TC>();
+    constexpr bool LTAB = col_ltab<T, NY, TPB, TC>();
     __shared__ cpx<T> lds[NY * TC];
-    __shared__ __attribute__((aligned(16))) cpx<T> ltw[LTAB ? NY : 1];   // e^{-2 pi i m/Ny}
+    __shared__ __attribute__((aligned(16))) cpx<T> ltw[col_tw_entries<T, NY, TPB, TC, LTAB>()];
     __shared__ __attribute__((aligned(16))) T lgy[LTAB ? NY : 1];        // exp(-ky^2/sigma^2)
     col_step_body<T, NY, TPB, TC, U, REFRESH, LTAB>(lds, ltw, lgy, blockIdx.x, J, K, Nh, I, fstride,
                                                     nstride, H, sI, sJ, twy_, ktab_, prm, n0);
