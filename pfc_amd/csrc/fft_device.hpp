@@ -227,6 +227,10 @@ __device__ __forceinline__ void gstore_real(GBuf b, unsigned voff, unsigned soff
 #ifndef PFC_LDS_SWIZZLE
 #define PFC_LDS_SWIZZLE 1
 #endif
+// FP64 transforms interleaved up to this many per LDS buffer are swizzled
+#ifndef PFC_SWZ_LS
+#define PFC_SWZ_LS 2
+#endif
 constexpr int ilog(int b, int n) { return n < b ? 0 : 1 + ilog(b, n / b); }
 constexpr int ipow(int b, int e) { return e == 0 ? 1 : b * ipow(b, e - 1); }
 
@@ -264,12 +268,12 @@ struct BlockFFT {
     // cycles per ideal cycle to ~0.1.  But each access then needs its own
     // address register instead of an immediate offset, and measured at 4096^2
     // with three fields that pays only where registers are to spare: FP64
-    // single-column transforms (column pass 447 -> 418 us).  FP32 and
-    // interleaved rows lost (mixed row pass 229 -> 309 us from spills, FP32
-    // column pass down to one block per CU, FP64 row pass +3.5 %), so the
-    // swizzle is on for FP64 with LS = 1 only.
+    // transforms, one column (column pass 447 -> 418 us) or two interleaved
+    // rows (row pass -3 %, 250 of 256 VGPRs, no spills).  FP32 lost (mixed
+    // row pass 229 -> 309 us from spills, FP32 column pass down to one block
+    // per CU), so the swizzle is on for FP64 with LS <= 2 only.
     static constexpr int LSB = (int)sizeof(cpx<T>) * LS;
-    static constexpr bool SWIZZLE = PFC_LDS_SWIZZLE && POW2 && sizeof(T) == 8 && LS == 1;
+    static constexpr bool SWIZZLE = PFC_LDS_SWIZZLE && POW2 && sizeof(T) == 8 && LS <= PFC_SWZ_LS;
     static constexpr int SWB = (!SWIZZLE || LSB >= 256) ? 0 : ilog(2, 256 / LSB);
     __device__ static __forceinline__ int sw(int i) {
         if constexpr (SWB == 0) {

@@ -4,7 +4,8 @@ swizzle and its composition from a per-lane base and per-access constants,
 for LS transforms interleaved in one buffer.  Each emulated transform must
 equal numpy's FFT, so a wrong stage index, a swizzle that is not a bijection
 or a base/offset pair that is not bit-disjoint fails here.  The device turns
-the swizzle on for FP64 single transforms (E = 16, LS = 1) only; the
+the swizzle on for FP64 with at most two interleaved transforms (E = 16,
+LS <= 2) only; the
 emulation also runs it on every other power-of-two shape, so the index
 algebra is checked wherever it could be turned on."""
 import math
@@ -25,7 +26,7 @@ class Plan:
         self.P = L // TPB
         self.pow2 = L & (L - 1) == 0
         if swizzle is None:   # the device's rule
-            swizzle = E == 16 and LS == 1
+            swizzle = E == 16 and LS <= 2
         lsb = E * LS
         self.SWB = 0 if (not swizzle or not self.pow2 or lsb >= 256) else ilog(2, 256 // lsb)
 
@@ -134,7 +135,7 @@ SHAPES = [
 @pytest.mark.parametrize("L,TPB,LS,E", SHAPES)
 def test_power_of_two_plans(L, TPB, LS, E, swizzle):
     p = check(L, TPB, LS, E, np.random.default_rng(L + TPB + LS + E), swizzle)
-    on = swizzle or (E == 16 and LS == 1)
+    on = swizzle or (E == 16 and LS <= 2)
     assert p.SWB == (0 if (not on or E * LS >= 256) else ilog(2, 256 // (E * LS)))
 
 
