@@ -434,10 +434,6 @@ __device__ __forceinline__ void nonlinear(const DevParams* __restrict__ p, T (&m
 
 // ================================================================ kernels
 
-// Row pass of a step.  Block = TR rows, lane = (fft thread t, row r) with the
-// row index fastest, so the TR rows' elements of one kx are adjacent lanes
-// and adjacent addresses in the column-major buffers (TR*sizeof(E) bytes
-// contiguous per kx).
 // Entries of the row pass's LDS twiddle table for L = 2^k row transforms:
 // the transform's compact stage table, then e^{-2 pi i t/Nx} for t < TPB
 // (the real-FFT pre/post twiddle), padded to a multiple of the block's
@@ -454,67 +450,48 @@ constexpr int row_tab_size() {
     return n ? (n + nt - 1) / nt * nt : 1;
 }
 
-// Body of the row pass for row group g (rows g*TR ... g*TR + TR - 1) on a
-// block of TPB*TR lanes; lds holds H*TR elements, ltab row_tab_size()
-// (16-byte aligned).  Shared by the one-system kernel below and the batch
-// kernels.
+// The row pass's table gather (L = 2^k): lane tid loads entries tid + i*NT
+// of the table from the Nx-point table (index clamped, no branch).
+template <typename T, int NX, int TPB, int TR, int NI>
+__device__ __forceinline__ void row_tab_gather(cpx<T> (&tv)[NI], GBuf twx, int tid) {
+    using F = BlockFFT<T, NX / 2, TPB, -1, 2, TR>;
+    constexpr int NT = TPB * TR, NTAB = row_tab_used<T, NX, TPB>();
+    constexpr unsigned E = sizeof(cpx<T>);
+    static_assert(NI * NT == row_tab_size<T, NX, TPB, TR>(), "one entry per lane and round");
+#pragma unroll
+    for (int i = 0; i < NI; i++) {
+        const int e = tid + i * NT;
+        const int m = e < F::kTab ? 2 * F::tab_src(e) : (e < NTAB ? e - F::kTab : 0);
+        tv[i] = gload<T>(twx, (unsigned)m * E, 0u);
+    }
+}
+
+// Row pass of a step.  Block = TR rows, lane = (fft thread t, row r) with the
+// row index fastest, so the TR rows' elements of one kx are adjacent lanes
+// and adjacent addresses in the column-major buffers (TR*sizeof(E) bytes
+// contiguous per kx).
 //
-// For L = 2^k the transforms take their twiddles from ltab, which each block
-// gathers from the Nx-point table before it loads the fields: global loads
-// and stores complete in issue order under one counter, so twiddle loads
-// issued during the transforms would make the C2R of field 0 wait for every
-// field's loads and the R2C of field u + 1 wait until field u's stores have
-// drained.
+// Core of the row pass once the U fields of row y are in registers (m[u][q]
+// = I_u(kx = t + q TPB, y)): C2R, NaN check, refresh R2C into K, nonlinear
+// term, R2C into J.  L = lds + r (H*TR elements); ltab: the twiddle table
+// (L = 2^k; complete before the first barrier of c2r_pre).
+//
+// For L = 2^k the transforms take their twiddles from ltab: global loads and
+// stores complete in issue order under one counter, so twiddle loads issued
+// during the transforms would make the C2R of field 0 wait for every field's
+// loads and the R2C of field u + 1 wait until field u's stores have drained.
 template <typename T, int NX, int TPB, int TR, int U, bool REFRESH>
-__device__ __forceinline__ void row_step_body(
-    cpx<T>* lds, cpx<T>* ltab, int g, const cpx<T>* __restrict__ I, cpx<T>* __restrict__ J,
-    cpx<T>* __restrict__ K, size_t fstride, int sI, int sJ, const cpx<T>* __restrict__ twx_,
+__device__ __forceinline__ void row_step_core(
+    cpx<T> (&m)[U][NX / 2 / TPB], cpx<T>* L, const cpx<T>* ltab, GBuf twx, int t, int y,
+    cpx<T>* __restrict__ J, cpx<T>* __restrict__ K, size_t fstride, int sJ,
     const DevParams* __restrict__ prm, int* __restrict__ nan_step, int tstep) {
     constexpr int H = NX / 2;
     constexpr int P = H / TPB;
-    constexpr int NT = TPB * TR;
     constexpr unsigned E = sizeof(cpx<T>);
     using FFTF = BlockFFT<T, H, TPB, -1, 2, TR>;
     using FFTI = BlockFFT<T, H, TPB, +1, 2, TR>;
     constexpr bool STAB = FFTF::POW2;
-    constexpr int NTAB = row_tab_used<T, NX, TPB>();
-    constexpr int NI = row_tab_size<T, NX, TPB, TR>() / NT;
-    const int tid = threadIdx.x;
-    const int r = tid % TR;
-    const int t = tid / TR;
-    const int y = g * TR + r;
-    cpx<T>* L = lds + r;
-    const GBuf twx = gbuf(twx_);
-    const unsigned ioff = lay_row_off(sI, H, t, y, E), istep = lay_row_step(sI, TPB, E);
     const unsigned voff = lay_row_off(sJ, H, t, y, E), qstep = lay_row_step(sJ, TPB, E);
-
-    PFC_PROBE_AT(0, 0);
-    cpx<T> tv[NI > 0 ? NI : 1];
-    if constexpr (STAB) {
-#pragma unroll
-        for (int i = 0; i < NI; i++) {
-            // unconditional (index clamped), so that no branch separates the
-            // gather from the field loads
-            const int e = tid + i * NT;
-            const int m = e < FFTF::kTab ? 2 * FFTF::tab_src(e) : (e < NTAB ? e - FFTF::kTab : 0);
-            tv[i] = gload<T>(twx, (unsigned)m * E, 0u);
-        }
-        __builtin_amdgcn_sched_barrier(0);   // issued before the field loads
-    }
-    cpx<T> m[U][P];
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-        const GBuf src = gbuf(I + u * fstride);
-#pragma unroll
-        for (int q = 0; q < P; q++) m[u][q] = gload<T>(src, ioff, q * istep);
-    }
-    if constexpr (STAB) {
-        // keep the table's LDS writes (which wait for its gather) after the
-        // field loads have been issued
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int i = 0; i < NI; i++) ltab[tid + i * NT] = tv[i];
-    }
     // the transforms' twiddles and the pre/post twiddle of lane t (ltab is
     // read only after the first barrier inside c2r_pre)
     const StageTab<T> stab{ltab};
@@ -580,6 +557,51 @@ __device__ __forceinline__ void row_step_body(
         for (int q = 0; q < P; q++) gstore<T, PFC_ROWJ_AUX>(dst, voff, q * qstep, m[u][q]);
     }
     PFC_PROBE_AT(0, 4);
+}
+
+// Body of the row pass for row group g (rows g*TR ... g*TR + TR - 1) on a
+// block of TPB*TR lanes; lds holds H*TR elements, ltab row_tab_size()
+// (16-byte aligned).  The table gather is issued before the field loads and
+// written to LDS after them, so that the write waits for the gather only.
+// Shared by the one-system kernel below and the batch kernels.
+template <typename T, int NX, int TPB, int TR, int U, bool REFRESH>
+__device__ __forceinline__ void row_step_body(
+    cpx<T>* lds, cpx<T>* ltab, int g, const cpx<T>* __restrict__ I, cpx<T>* __restrict__ J,
+    cpx<T>* __restrict__ K, size_t fstride, int sI, int sJ, const cpx<T>* __restrict__ twx_,
+    const DevParams* __restrict__ prm, int* __restrict__ nan_step, int tstep) {
+    constexpr int H = NX / 2;
+    constexpr int P = H / TPB;
+    constexpr int NT = TPB * TR;
+    constexpr unsigned E = sizeof(cpx<T>);
+    constexpr bool STAB = BlockFFT<T, H, TPB, -1, 2, TR>::POW2;
+    constexpr int NI = row_tab_size<T, NX, TPB, TR>() / NT;
+    const int tid = threadIdx.x;
+    const int r = tid % TR;
+    const int t = tid / TR;
+    const int y = g * TR + r;
+    const GBuf twx = gbuf(twx_);
+    const unsigned ioff = lay_row_off(sI, H, t, y, E), istep = lay_row_step(sI, TPB, E);
+
+    PFC_PROBE_AT(0, 0);
+    cpx<T> tv[NI > 0 ? NI : 1];
+    if constexpr (STAB) {
+        row_tab_gather<T, NX, TPB, TR>(tv, twx, tid);
+        __builtin_amdgcn_sched_barrier(0);   // issued before the field loads
+    }
+    cpx<T> m[U][P];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const GBuf src = gbuf(I + u * fstride);
+#pragma unroll
+        for (int q = 0; q < P; q++) m[u][q] = gload<T>(src, ioff, q * istep);
+    }
+    if constexpr (STAB) {
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int i = 0; i < NI; i++) ltab[tid + i * NT] = tv[i];
+    }
+    row_step_core<T, NX, TPB, TR, U, REFRESH>(m, lds + r, ltab, twx, t, y, J, K, fstride, sJ, prm,
+                                              nan_step, tstep);
 }
 
 template <typename T, int NX, int TPB, int TR, int U, bool REFRESH>
