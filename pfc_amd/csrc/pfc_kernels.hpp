@@ -54,8 +54,17 @@ constexpr int kMaxU = 4;   // fields supported by the fused kernels
 #ifndef PFC_COL_WAVES_F32
 #define PFC_COL_WAVES_F32 2
 #endif
-template <typename T>
-constexpr int row_waves() { return sizeof(T) == 8 ? PFC_ROW_WAVES_F64 : PFC_ROW_WAVES_F32; }
+// FP32 rows of 4096 points run 2 rows per block at 4 waves per SIMD, so that
+// two blocks share a CU and one block's loads and stores overlap the other's
+// transforms (the kernel needs 118 VGPRs; mixed step 383 -> 363 us, though
+// the rows' 16-byte runs copy slower than 4 rows' 32-byte runs)
+#ifndef PFC_ROW_WAVES_F32_4096
+#define PFC_ROW_WAVES_F32_4096 4
+#endif
+template <typename T, int NX = 0>
+constexpr int row_waves() {
+    return sizeof(T) == 8 ? PFC_ROW_WAVES_F64 : (NX == 4096 ? PFC_ROW_WAVES_F32_4096 : PFC_ROW_WAVES_F32);
+}
 template <typename T>
 constexpr int col_waves() { return sizeof(T) == 8 ? PFC_COL_WAVES_F64 : PFC_COL_WAVES_F32; }
 
@@ -605,7 +614,7 @@ __device__ __forceinline__ void row_step_body(
 }
 
 template <typename T, int NX, int TPB, int TR, int U, bool REFRESH>
-__global__ __launch_bounds__(TPB* TR) PFC_WAVES(row_waves<T>()) void k_row_step(
+__global__ __launch_bounds__(TPB* TR) PFC_WAVES((row_waves<T, NX>())) void k_row_step(
     const cpx<T>* __restrict__ I, cpx<T>* __restrict__ J, cpx<T>* __restrict__ K,
     size_t fstride, int sI, int sJ, const cpx<T>* __restrict__ twx_,
     const DevParams* __restrict__ prm, int* __restrict__ nan_step, int tstep) {

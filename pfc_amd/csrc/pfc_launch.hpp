@@ -25,7 +25,7 @@ constexpr bool size_built(int n) { return n >= PFC_NMIN && n <= PFC_NMAX; }
 #define PFC_ROW_TR4096_F64 2
 #endif
 #ifndef PFC_ROW_TR4096_F32
-#define PFC_ROW_TR4096_F32 4
+#define PFC_ROW_TR4096_F32 2
 #endif
 
 struct RowCfg {
