@@ -83,7 +83,7 @@ __global__ __launch_bounds__(N / 2 / PP * batch_tr<T>(N, PP)) PFC_WAVES(row_wave
     const BatchItem* __restrict__ items, int nitems, const BatchSys<T>* __restrict__ sys,
     const int* __restrict__ tdev, int toff) {
     constexpr int TPB = N / 2 / PP, TR = batch_tr<T>(N, PP);
-    __shared__ cpx<T> lds[N / 2 * TR];
+    __shared__ cpx<T> lds[row_lds_elems<T, N, TPB, TR>()];
     __shared__ __attribute__((aligned(16))) cpx<T> ltab[row_tab_size<T, N, TPB, TR>()];
     const int tstep = *tdev + toff;   // the step's index (NaN reports)
     if ((int)blockIdx.x >= nitems) return;
@@ -99,7 +99,7 @@ __global__ __launch_bounds__(N / PP * batch_tc<T>(N, PP)) PFC_WAVES(col_waves<T>
     const BatchItem* __restrict__ items, int nitems, const BatchSys<T>* __restrict__ sys) {
     constexpr int TPB = N / PP, TC = batch_tc<T>(N, PP);
     constexpr bool LTAB = col_ltab<T, N, TPB, TC>();
-    __shared__ cpx<T> lds[N * TC];
+    __shared__ cpx<T> lds[col_lds_len<T, N, TPB>() * TC];
     __shared__ __attribute__((aligned(16))) cpx<T> ltw[col_tw_entries<T, N, TPB, TC, LTAB>()];
     __shared__ __attribute__((aligned(16))) T lgy[LTAB ? N : 1];
     if ((int)blockIdx.x >= nitems) return;

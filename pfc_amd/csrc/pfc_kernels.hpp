@@ -42,6 +42,9 @@ constexpr int kMaxU = 4;   // fields supported by the fused kernels
 #ifndef PFC_COL_AUX
 #define PFC_COL_AUX 2
 #endif
+#ifndef PFC_STEP_PADC
+#define PFC_STEP_PADC 3
+#endif
 #ifndef PFC_ROW_WAVES_F64
 #define PFC_ROW_WAVES_F64 2
 #endif
@@ -443,6 +446,19 @@ __device__ __forceinline__ void nonlinear(const DevParams* __restrict__ p, T (&m
 
 // ================================================================ kernels
 
+// LDS index padding of the step kernels' FP32 transforms (BlockFFT PADC;
+// FP64 transforms are swizzled instead), and the exchange buffers' sizes.
+template <typename T>
+constexpr int step_padc() { return sizeof(T) == 4 ? PFC_STEP_PADC : 0; }
+template <typename T, int NX, int TPB, int TR>
+constexpr int row_lds_elems() {
+    return BlockFFT<T, NX / 2, TPB, -1, 2, TR, step_padc<T>()>::lds_len * TR;
+}
+template <typename T, int NY, int TPB>
+constexpr int col_lds_len() {
+    return BlockFFT<T, NY, TPB, -1, 1, 1, step_padc<T>()>::lds_len;
+}
+
 // Entries of the row pass's LDS twiddle table for L = 2^k row transforms:
 // the transform's compact stage table, then e^{-2 pi i t/Nx} for t < TPB
 // (the real-FFT pre/post twiddle), padded to a multiple of the block's
@@ -497,8 +513,8 @@ __device__ __forceinline__ void row_step_core(
     constexpr int H = NX / 2;
     constexpr int P = H / TPB;
     constexpr unsigned E = sizeof(cpx<T>);
-    using FFTF = BlockFFT<T, H, TPB, -1, 2, TR>;
-    using FFTI = BlockFFT<T, H, TPB, +1, 2, TR>;
+    using FFTF = BlockFFT<T, H, TPB, -1, 2, TR, step_padc<T>()>;
+    using FFTI = BlockFFT<T, H, TPB, +1, 2, TR, step_padc<T>()>;
     constexpr bool STAB = FFTF::POW2;
     const unsigned voff = lay_row_off(sJ, H, t, y, E), qstep = lay_row_step(sJ, TPB, E);
     // the transforms' twiddles and the pre/post twiddle of lane t (ltab is
@@ -618,7 +634,7 @@ __global__ __launch_bounds__(TPB* TR) PFC_WAVES((row_waves<T, NX>())) void k_row
     const cpx<T>* __restrict__ I, cpx<T>* __restrict__ J, cpx<T>* __restrict__ K,
     size_t fstride, int sI, int sJ, const cpx<T>* __restrict__ twx_,
     const DevParams* __restrict__ prm, int* __restrict__ nan_step, int tstep) {
-    __shared__ cpx<T> lds[NX / 2 * TR];
+    __shared__ cpx<T> lds[row_lds_elems<T, NX, TPB, TR>()];
     __shared__ __attribute__((aligned(16))) cpx<T> ltab[row_tab_size<T, NX, TPB, TR>()];
 #ifndef PFC_ROW_XCD
 #define PFC_ROW_XCD 1
@@ -719,12 +735,12 @@ __device__ __forceinline__ void col_step_body(
     const T* __restrict__ ktab_, const DevParams* __restrict__ prm, double* __restrict__ n0) {
     constexpr int P = NY / TPB;
     constexpr unsigned E = sizeof(cpx<T>);
-    using FFTF = BlockFFT<T, NY, TPB, -1, 1, 1>;
-    using FFTI = BlockFFT<T, NY, TPB, +1, 1, 1>;
+    using FFTF = BlockFFT<T, NY, TPB, -1, 1, 1, step_padc<T>()>;
+    using FFTI = BlockFFT<T, NY, TPB, +1, 1, 1, step_padc<T>()>;
     const int tid = threadIdx.x;
     const int c = tid / TPB;
     const int t = tid % TPB;
-    cpx<T>* L = lds + c * NY;
+    cpx<T>* L = lds + c * FFTF::lds_len;
     const GBuf twy = gbuf(twy_);
     const GBuf ktab = gbuf(ktab_);
     PFC_PROBE_AT(65536, 0);
@@ -903,7 +919,7 @@ __device__ __forceinline__ void col_step_body(
 // buffer (all sizes but FP64 Ny = 8192)
 template <typename T, int NY, int TPB, int TC>
 constexpr bool col_ltab() {
-    return NY * TC * sizeof(cpx<T>) + col_tw_entries<T, NY, TPB, TC, true>() * sizeof(cpx<T>) +
+    return col_lds_len<T, NY, TPB>() * TC * sizeof(cpx<T>) + col_tw_entries<T, NY, TPB, TC, true>() * sizeof(cpx<T>) +
                NY * sizeof(T) <= 160 * 1024;
 }
 
@@ -914,7 +930,7 @@ __global__ __launch_bounds__(TPB* TC) PFC_WAVES(col_waves<T>()) void k_col_step(
     const cpx<T>* __restrict__ twy_, const T* __restrict__ ktab_,
     const DevParams* __restrict__ prm, double* __restrict__ n0) {
     constexpr bool LTAB = col_ltab<T, NY, TPB, TC>();
-    __shared__ cpx<T> lds[NY * TC];
+    __shared__ cpx<T> lds[col_lds_len<T, NY, TPB>() * TC];
     __shared__ __attribute__((aligned(16))) cpx<T> ltw[col_tw_entries<T, NY, TPB, TC, LTAB>()];
     __shared__ __attribute__((aligned(16))) T lgy[LTAB ? NY : 1];        // exp(-ky^2/sigma^2)
     col_step_body<T, NY, TPB, TC, U, REFRESH, LTAB>(lds, ltw, lgy, blockIdx.x, J, K, Nh, I, fstride,

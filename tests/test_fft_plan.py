@@ -21,7 +21,7 @@ def ilog(b, n):
 class Plan:
     """Index arithmetic of BlockFFT<T, L, TPB, DIR, TWS, LS> (E = sizeof(cpx<T>))."""
 
-    def __init__(self, L, TPB, LS, E, swizzle=None):
+    def __init__(self, L, TPB, LS, E, swizzle=None, padc=0):
         self.L, self.TPB, self.LS, self.E = L, TPB, LS, E
         self.P = L // TPB
         self.pow2 = L & (L - 1) == 0
@@ -29,6 +29,11 @@ class Plan:
             swizzle = E == 16 and LS <= 2
         lsb = E * LS
         self.SWB = 0 if (not swizzle or not self.pow2 or lsb >= 256) else ilog(2, 256 // lsb)
+        self.PC = padc if (self.pow2 and self.SWB == 0) else 0
+        self.lds_len = L + self.PC * ((L - 1) >> 5)
+
+    def pd(self, i):
+        return i + self.PC * (i >> 5)
 
     def sw(self, i):
         if self.SWB == 0:
@@ -55,7 +60,7 @@ class Plan:
         t = np.arange(TPB)
         # thread t holds elements t + q TPB (natural order)
         v = np.stack([x[:, t + q * TPB] for q in range(P)], axis=2)   # (LS, TPB, P)
-        lds = np.full(L * LS, np.nan, dtype=complex)
+        lds = np.full(self.lds_len * LS, np.nan, dtype=complex)
         rr = np.arange(LS)[:, None]
         rads = self.radices()
         Ns = 1
@@ -77,13 +82,17 @@ class Plan:
                         nv[:, :, b + r * B] = X[r]
                 else:
                     logical = (j - k) * R + k
-                    base = np.array([self.sw(int(a)) for a in logical])
+                    if self.PC:
+                        base = self.pd(logical)
+                    else:
+                        base = np.array([self.sw(int(a)) for a in logical])
                     for r in range(R):
                         if self.SWB:
                             assert not np.any(logical & (r * Ns))
                             idx = base ^ self.sw(r * Ns)
                         else:
-                            idx = base + r * Ns
+                            idx = base + self.pd(r * Ns)
+                            assert np.array_equal(idx, self.pd(logical + r * Ns))
                         lds[idx[None, :] * LS + rr] = X[r]
             if last:
                 v = nv
@@ -93,13 +102,14 @@ class Plan:
             Bn = P // Rn
             for b in range(Bn):
                 j = t + b * TPB
-                jb = np.array([self.sw(int(a)) for a in j])
+                jb = self.pd(j) if self.PC else np.array([self.sw(int(a)) for a in j])
                 for r in range(Rn):
                     if self.SWB:
                         assert not np.any(j & (r * (L // Rn)))
                         idx = jb ^ self.sw(r * (L // Rn))
                     else:
-                        idx = jb + r * (L // Rn)
+                        idx = jb + self.pd(r * (L // Rn))
+                        assert np.array_equal(idx, self.pd(j + r * (L // Rn)))
                     v[:, :, b + r * Bn] = lds[idx[None, :] * LS + rr]
         out = np.empty_like(x)
         for q in range(P):
@@ -107,8 +117,8 @@ class Plan:
         return out
 
 
-def check(L, TPB, LS, E, rng, swizzle=None):
-    p = Plan(L, TPB, LS, E, swizzle)
+def check(L, TPB, LS, E, rng, swizzle=None, padc=0):
+    p = Plan(L, TPB, LS, E, swizzle, padc)
     x = rng.standard_normal((LS, L)) + 1j * rng.standard_normal((LS, L))
     ref = np.fft.fft(x, axis=1)
     scale = np.abs(ref).max()
@@ -222,3 +232,13 @@ def test_row_partner_index(H, TPB):
         for t in range(TPB):
             k = t + q * TPB
             assert row_partner(t, q) == (H - k) % H
+
+
+@pytest.mark.parametrize("L,TPB,LS", [(L, T, LS) for (L, T, LS, E) in SHAPES if E == 8] +
+                         [(2048, 256, 2), (4096, 1024, 2), (1024, 64, 4), (256, 16, 16)])
+def test_padded_fp32_plans(L, TPB, LS):
+    """The FP32 step transforms' LDS padding (index i at i + 3 (i >> 5)): every
+    access's base + immediate form equals the padded index (asserted inside),
+    the transform is still numpy's, and the buffer holds lds_len * LS."""
+    p = check(L, TPB, LS, 8, np.random.default_rng(L * LS + 1), padc=3)
+    assert p.PC == 3 and p.lds_len == L + 3 * ((L - 1) >> 5)
