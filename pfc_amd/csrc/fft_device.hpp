@@ -273,7 +273,8 @@ struct BlockFFT {
     // row pass 229 -> 309 us from spills, FP32 column pass down to one block
     // per CU), so the swizzle is on for FP64 with LS <= 2 only.
     static constexpr int LSB = (int)sizeof(cpx<T>) * LS;
-    static constexpr bool SWIZZLE = PFC_LDS_SWIZZLE && POW2 && sizeof(T) == 8 && LS <= PFC_SWZ_LS;
+    static constexpr bool SWIZZLE =
+        PFC_LDS_SWIZZLE && POW2 && sizeof(T) == 8 && LS <= PFC_SWZ_LS && PADC == 0;
     static constexpr int SWB = (!SWIZZLE || LSB >= 256) ? 0 : ilog(2, 256 / LSB);
     __device__ static __forceinline__ int sw(int i) {
         if constexpr (SWB == 0) {
@@ -284,18 +285,20 @@ struct BlockFFT {
         }
     }
 
-    // LDS index padding (L = 2^k, when not swizzled): index i is stored at
-    // i + PADC * (i >> 5).  The pad is additive over the same base/offset
-    // split as the swizzle (a lane's base and an access's constant never carry
-    // across a multiple of 32: the constant r Ns is below Ns R with Ns | 32 or
-    // a multiple of 32), so an access stays base + immediate offset and needs
-    // no extra register.  PADC = 3 brings the modelled write conflicts of the
-    // FP32 step transforms from 1.0 (rows) and 1.67 (columns) extra cycles per
+    // LDS index padding (L = 2^k, instead of the swizzle): index i is stored
+    // at i + PADC * (i >> PSH), PSH = log2(elements per 256-byte bank row).
+    // The pad is additive over the same base/offset split as the swizzle (a
+    // lane's base and an access's constant never carry across a multiple of
+    // 2^PSH: the constant r Ns is below Ns R with Ns | 2^PSH, or a multiple of
+    // it), so an access stays base + immediate offset and needs no extra
+    // register.  FP32 PADC = 3 brings the modelled write conflicts of the step
+    // transforms from 1.0 (rows) and 1.67 (columns) extra cycles per
     // conflict-free cycle to 0.33 and 0.17, for 9.4 % more exchange LDS.
+    static constexpr int PSH = ilog(2, 256 / (int)sizeof(cpx<T>));
     static constexpr int PC = (POW2 && SWB == 0) ? PADC : 0;
-    __device__ static __forceinline__ int pd(int i) { return PC ? i + PC * (i >> 5) : i; }
+    __device__ static __forceinline__ int pd(int i) { return PC ? i + PC * (i >> PSH) : i; }
     // elements one transform's exchange occupies (times LS when interleaved)
-    static constexpr int lds_len = PC ? L + PC * ((L - 1) >> 5) : L;
+    static constexpr int lds_len = PC ? L + PC * ((L - 1) >> PSH) : L;
 
     // L = 3 * 2^k: stage s has radix gcd(P, L / (radices of stages < s)), so
     // the first stage takes the factor 3 (radix 12 or 6), the rest radix 4 or 2.

@@ -45,6 +45,9 @@ constexpr int kMaxU = 4;   // fields supported by the fused kernels
 #ifndef PFC_STEP_PADC
 #define PFC_STEP_PADC 3
 #endif
+#ifndef PFC_STEP_PADC64
+#define PFC_STEP_PADC64 0
+#endif
 #ifndef PFC_ROW_WAVES_F64
 #define PFC_ROW_WAVES_F64 2
 #endif
@@ -449,7 +452,7 @@ __device__ __forceinline__ void nonlinear(const DevParams* __restrict__ p, T (&m
 // LDS index padding of the step kernels' FP32 transforms (BlockFFT PADC;
 // FP64 transforms are swizzled instead), and the exchange buffers' sizes.
 template <typename T>
-constexpr int step_padc() { return sizeof(T) == 4 ? PFC_STEP_PADC : 0; }
+constexpr int step_padc() { return sizeof(T) == 4 ? PFC_STEP_PADC : PFC_STEP_PADC64; }
 template <typename T, int NX, int TPB, int TR>
 constexpr int row_lds_elems() {
     return BlockFFT<T, NX / 2, TPB, -1, 2, TR, step_padc<T>()>::lds_len * TR;

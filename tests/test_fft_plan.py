@@ -26,14 +26,15 @@ class Plan:
         self.P = L // TPB
         self.pow2 = L & (L - 1) == 0
         if swizzle is None:   # the device's rule
-            swizzle = E == 16 and LS <= 2
+            swizzle = E == 16 and LS <= 2 and padc == 0
         lsb = E * LS
         self.SWB = 0 if (not swizzle or not self.pow2 or lsb >= 256) else ilog(2, 256 // lsb)
         self.PC = padc if (self.pow2 and self.SWB == 0) else 0
-        self.lds_len = L + self.PC * ((L - 1) >> 5)
+        self.PSH = ilog(2, 256 // E)
+        self.lds_len = L + self.PC * ((L - 1) >> self.PSH)
 
     def pd(self, i):
-        return i + self.PC * (i >> 5)
+        return i + self.PC * (i >> self.PSH)
 
     def sw(self, i):
         if self.SWB == 0:
@@ -242,3 +243,10 @@ def test_padded_fp32_plans(L, TPB, LS):
     the transform is still numpy's, and the buffer holds lds_len * LS."""
     p = check(L, TPB, LS, 8, np.random.default_rng(L * LS + 1), padc=3)
     assert p.PC == 3 and p.lds_len == L + 3 * ((L - 1) >> 5)
+
+
+@pytest.mark.parametrize("L,TPB,LS", [(L, T, LS) for (L, T, LS, E) in SHAPES if E == 16])
+def test_padded_fp64_plans(L, TPB, LS):
+    """The FP64 alternative to the swizzle (index i at i + 2 (i >> 4))."""
+    p = check(L, TPB, LS, 16, np.random.default_rng(L * LS + 2), padc=2)
+    assert p.SWB == 0 and p.PC == 2 and p.lds_len == L + 2 * ((L - 1) >> 4)
