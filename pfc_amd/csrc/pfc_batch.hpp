@@ -79,7 +79,7 @@ constexpr int batch_tc(int c) { return batch_tc<T>(kBatchClass[c].n, kBatchClass
 
 // Row pass of one class: block w runs item w.
 template <typename T, int N, int PP, int U, bool REF>
-__global__ __launch_bounds__(N / 2 / PP * batch_tr<T>(N, PP)) PFC_WAVES(row_waves<T>()) void k_batch_row(
+__global__ __launch_bounds__(N / 2 / PP * batch_tr<T>(N, PP)) PFC_WAVES((row_waves<T, N>())) void k_batch_row(
     const BatchItem* __restrict__ items, int nitems, const BatchSys<T>* __restrict__ sys,
     const int* __restrict__ tdev, int toff) {
     constexpr int TPB = N / 2 / PP, TR = batch_tr<T>(N, PP);
