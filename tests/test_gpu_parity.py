@@ -72,6 +72,22 @@ def test_wide_and_tall_grids(gpu, Nx, Ny, precision):
 
 
 @pytest.mark.parametrize("precision", [FP64, MIXED])
+@pytest.mark.parametrize("U", [1, 2])
+@pytest.mark.parametrize("Nx,Ny", [(4096, 64), (2048, 128)])
+def test_long_rows_one_and_two_fields(gpu, Nx, Ny, U, precision):
+    """The 4096- and 2048-point row kernels (2 or 4 FP32 rows per block, two
+    blocks per CU) with one and two fields, across the refresh step t = 100."""
+    if U == 1:
+        p, f = graphene_params(), graphene_state(Nx, Ny, seed=21)
+    else:
+        p = two_field_params(c=1.0)
+        f = np.concatenate([graphene_state(Nx, Ny, seed=22), graphene_state(Nx, Ny, seed=23)])
+    o, g = run_pair(p, f, Nx, Ny, DX, DX, 98, 4, precision=precision)
+    err = rel_l2(g.get_fields(), o.get_fields())
+    assert err < (1e-13 if precision == FP64 else 2e-6), err
+
+
+@pytest.mark.parametrize("precision", [FP64, MIXED])
 @pytest.mark.parametrize("Nx,Ny", [(384, 384), (768, 384), (384, 1536), (1536, 768), (768, 64),
                                    (64, 1536)])
 def test_three_times_power_of_two_grids(gpu, Nx, Ny, precision):
