@@ -71,9 +71,12 @@ constexpr int kMaxU = 4;   // fields supported by the fused kernels
 #ifndef PFC_ROW_WAVES_F32_2048
 #define PFC_ROW_WAVES_F32_2048 4
 #endif
+#ifndef PFC_ROW_WAVES_F64_4096
+#define PFC_ROW_WAVES_F64_4096 2
+#endif
 template <typename T, int NX = 0>
 constexpr int row_waves() {
-    return sizeof(T) == 8 ? PFC_ROW_WAVES_F64
+    return sizeof(T) == 8 ? (NX == 4096 ? PFC_ROW_WAVES_F64_4096 : PFC_ROW_WAVES_F64)
                           : (NX == 4096 ? PFC_ROW_WAVES_F32_4096
                                         : (NX == 2048 ? PFC_ROW_WAVES_F32_2048 : PFC_ROW_WAVES_F32));
 }

@@ -24,6 +24,9 @@ constexpr bool size_built(int n) { return n >= PFC_NMIN && n <= PFC_NMAX; }
 #ifndef PFC_ROW_TR4096_F64
 #define PFC_ROW_TR4096_F64 2
 #endif
+#ifndef PFC_ROW_TPB4096_F64
+#define PFC_ROW_TPB4096_F64 256
+#endif
 #ifndef PFC_ROW_TR4096_F32
 #define PFC_ROW_TR4096_F32 2
 #endif
@@ -51,7 +54,7 @@ constexpr RowCfg row_cfg(int nx) {
         case 1024: return {64, 4};
         case 1536: return {64, 4};
         case 2048: return {128, f64 ? 4 : PFC_ROW_TR2048_F32};
-        case 4096: return {256, f64 ? PFC_ROW_TR4096_F64 : PFC_ROW_TR4096_F32};
+        case 4096: return {f64 ? PFC_ROW_TPB4096_F64 : 256, f64 ? PFC_ROW_TR4096_F64 : PFC_ROW_TR4096_F32};
         case 8192: return {512, f64 ? 1 : 2};
         default: return {0, 0};
     }
