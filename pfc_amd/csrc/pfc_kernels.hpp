@@ -60,7 +60,7 @@ constexpr int kMaxU = 4;   // fields supported by the fused kernels
 #ifndef PFC_COL_WAVES_F32
 #define PFC_COL_WAVES_F32 2
 #endif
-// FP32 rows of 4096 (2048) points run 2 (4) rows per block at 4 waves per
+// FP32 rows of 4096 (2048, 8192) points run 2 (4, 1) rows per block at 4 waves per
 // SIMD, so that two blocks share a CU and one block's loads and stores
 // overlap the other's transforms (the 4096-point kernel needs 123 VGPRs;
 // mixed step 383 -> 363 us, though 2 rows' 16-byte runs copy slower than 4
@@ -71,14 +71,19 @@ constexpr int kMaxU = 4;   // fields supported by the fused kernels
 #ifndef PFC_ROW_WAVES_F32_2048
 #define PFC_ROW_WAVES_F32_2048 4
 #endif
+#ifndef PFC_ROW_WAVES_F32_8192
+#define PFC_ROW_WAVES_F32_8192 4
+#endif
 #ifndef PFC_ROW_WAVES_F64_4096
 #define PFC_ROW_WAVES_F64_4096 2
 #endif
 template <typename T, int NX = 0>
 constexpr int row_waves() {
     return sizeof(T) == 8 ? (NX == 4096 ? PFC_ROW_WAVES_F64_4096 : PFC_ROW_WAVES_F64)
-                          : (NX == 4096 ? PFC_ROW_WAVES_F32_4096
-                                        : (NX == 2048 ? PFC_ROW_WAVES_F32_2048 : PFC_ROW_WAVES_F32));
+                          : (NX == 4096   ? PFC_ROW_WAVES_F32_4096
+                             : NX == 2048 ? PFC_ROW_WAVES_F32_2048
+                             : NX == 8192 ? PFC_ROW_WAVES_F32_8192
+                                          : PFC_ROW_WAVES_F32);
 }
 template <typename T>
 constexpr int col_waves() { return sizeof(T) == 8 ? PFC_COL_WAVES_F64 : PFC_COL_WAVES_F32; }

@@ -33,6 +33,9 @@ constexpr bool size_built(int n) { return n >= PFC_NMIN && n <= PFC_NMAX; }
 #ifndef PFC_ROW_TR2048_F32
 #define PFC_ROW_TR2048_F32 4
 #endif
+#ifndef PFC_ROW_TR8192_F32
+#define PFC_ROW_TR8192_F32 1
+#endif
 
 struct RowCfg {
     int tpb, tr;
@@ -55,7 +58,7 @@ constexpr RowCfg row_cfg(int nx) {
         case 1536: return {64, 4};
         case 2048: return {128, f64 ? 4 : PFC_ROW_TR2048_F32};
         case 4096: return {f64 ? PFC_ROW_TPB4096_F64 : 256, f64 ? PFC_ROW_TR4096_F64 : PFC_ROW_TR4096_F32};
-        case 8192: return {512, f64 ? 1 : 2};
+        case 8192: return {512, f64 ? 1 : PFC_ROW_TR8192_F32};
         default: return {0, 0};
     }
 }
