@@ -99,15 +99,17 @@ __global__ __launch_bounds__(N / PP * batch_tc<T>(N, PP)) PFC_WAVES(col_waves<T>
     const BatchItem* __restrict__ items, int nitems, const BatchSys<T>* __restrict__ sys) {
     constexpr int TPB = N / PP, TC = batch_tc<T>(N, PP);
     constexpr bool LTAB = col_ltab<T, N, TPB, TC>();
+    constexpr bool LGY = col_lgy<T, N, TPB, TC>();
+    static_assert(LGY, "batch lengths keep their ky factors in LDS");
     __shared__ cpx<T> lds[col_lds_len<T, N, TPB>() * TC];
     __shared__ __attribute__((aligned(16))) cpx<T> ltw[col_tw_entries<T, N, TPB, TC, LTAB>()];
-    __shared__ __attribute__((aligned(16))) T lgy[LTAB ? N : 1];
+    __shared__ __attribute__((aligned(16))) T lgy[N];
     if ((int)blockIdx.x >= nitems) return;
     const BatchItem it = items[blockIdx.x];
     const BatchSys<T>& S = sys[it.sys];
-    col_step_body<T, N, TPB, TC, U, REF, LTAB>(lds, ltw, lgy, it.blk, S.J, S.K, S.Nh, S.I, S.fstride,
-                                               S.nstride, S.H, S.sI, S.sJ, S.twy, S.ktab, S.prm,
-                                               S.n0);
+    col_step_body<T, N, TPB, TC, U, REF, LTAB, LGY>(lds, ltw, lgy, it.blk, S.J, S.K, S.Nh, S.I,
+                                                    S.fstride, S.nstride, S.H, S.sI, S.sJ, S.twy,
+                                                    S.ktab, S.prm, S.n0);
 }
 
 // advances the device step counter read by the row pass by the steps just run

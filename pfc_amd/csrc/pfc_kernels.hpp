@@ -741,10 +741,11 @@ constexpr int col_tw_entries() {
 #define PFC_COL_QB_F32 2
 #endif
 // Body of the column pass for column block cb (H/TC + 1 per system) on a
-// block of TPB*TC lanes.  lds holds NY*TC elements; with LTAB, ltw (NY
-// elements, 16-byte aligned) and lgy (NY reals) hold the twiddle and ky-factor
-// tables.  Shared by the one-system kernel below and the batch kernels.
-template <typename T, int NY, int TPB, int TC, int U, bool REFRESH, bool LTAB>
+// block of TPB*TC lanes.  lds holds col_lds_len * TC elements; with LTAB, ltw
+// (col_tw_entries, 16-byte aligned) holds the twiddle table, and with LGY,
+// lgy (NY reals) the ky factors (read from global memory otherwise).  Shared
+// by the one-system kernel below and the batch kernels.
+template <typename T, int NY, int TPB, int TC, int U, bool REFRESH, bool LTAB, bool LGY>
 __device__ __forceinline__ void col_step_body(
     cpx<T>* lds, cpx<T>* ltw, T* lgy, int cb, const cpx<T>* __restrict__ J,
     const cpx<T>* __restrict__ K, cpx<T>* __restrict__ Nh, cpx<T>* __restrict__ I, size_t fstride,
@@ -781,7 +782,8 @@ __device__ __forceinline__ void col_step_body(
         cpx<T> tv[NI];
         if constexpr (LTAB) {
             if constexpr (!CT) lds_copy16<NY * (int)E, NT>(twy, 0u, ltw, tid);
-            lds_copy16<NY * (int)sizeof(T), NT>(ktab, (unsigned)ktab_gy(H) * sizeof(T), lgy, tid);
+            if constexpr (LGY)
+                lds_copy16<NY * (int)sizeof(T), NT>(ktab, (unsigned)ktab_gy(H) * sizeof(T), lgy, tid);
         }
         if constexpr (CT) {
 #pragma unroll
@@ -866,7 +868,7 @@ __device__ __forceinline__ void col_step_body(
                     const double ky = ky_of(j, NY, prm->dky);
                     const T k2 = (T)(kx2 + ky * ky);
                     T gy;
-                    if constexpr (LTAB) gy = lgy[j];
+                    if constexpr (LGY) gy = lgy[j];
                     else gy = gload_real<T>(ktab, (unsigned)(ktab_gy(H) + j) * sizeof(T));
                     const T G2 = gx * gy;
                     cpx<T> Mk[U];
@@ -932,12 +934,19 @@ __device__ __forceinline__ void col_step_body(
     PFC_PROBE_AT(65536, 4);
 }
 
-// twiddle and ky-factor tables in LDS where they fit beside the exchange
-// buffer (all sizes but FP64 Ny = 8192)
+// twiddle table in LDS where it fits beside the exchange buffer (all sizes),
+// and the ky factors too where they also fit (all but FP64 Ny = 8192, where
+// the 128 KB exchange buffer and the 32 KB compact table fill the 160 KB)
 template <typename T, int NY, int TPB, int TC>
 constexpr bool col_ltab() {
-    return col_lds_len<T, NY, TPB>() * TC * sizeof(cpx<T>) + col_tw_entries<T, NY, TPB, TC, true>() * sizeof(cpx<T>) +
-               NY * sizeof(T) <= 160 * 1024;
+    return col_lds_len<T, NY, TPB>() * TC * sizeof(cpx<T>) +
+               col_tw_entries<T, NY, TPB, TC, true>() * sizeof(cpx<T>) <= 160 * 1024;
+}
+template <typename T, int NY, int TPB, int TC>
+constexpr bool col_lgy() {
+    return col_ltab<T, NY, TPB, TC>() &&
+           col_lds_len<T, NY, TPB>() * TC * sizeof(cpx<T>) +
+                   col_tw_entries<T, NY, TPB, TC, true>() * sizeof(cpx<T>) + NY * sizeof(T) <= 160 * 1024;
 }
 
 template <typename T, int NY, int TPB, int TC, int U, bool REFRESH>
@@ -947,11 +956,19 @@ __global__ __launch_bounds__(TPB* TC) PFC_WAVES(col_waves<T>()) void k_col_step(
     const cpx<T>* __restrict__ twy_, const T* __restrict__ ktab_,
     const DevParams* __restrict__ prm, double* __restrict__ n0) {
     constexpr bool LTAB = col_ltab<T, NY, TPB, TC>();
+    constexpr bool LGY = col_lgy<T, NY, TPB, TC>();
     __shared__ cpx<T> lds[col_lds_len<T, NY, TPB>() * TC];
     __shared__ __attribute__((aligned(16))) cpx<T> ltw[col_tw_entries<T, NY, TPB, TC, LTAB>()];
-    __shared__ __attribute__((aligned(16))) T lgy[LTAB ? NY : 1];        // exp(-ky^2/sigma^2)
-    col_step_body<T, NY, TPB, TC, U, REFRESH, LTAB>(lds, ltw, lgy, blockIdx.x, J, K, Nh, I, fstride,
-                                                    nstride, H, sI, sJ, twy_, ktab_, prm, n0);
+    if constexpr (LGY) {
+        __shared__ __attribute__((aligned(16))) T lgy[NY];   // exp(-ky^2/sigma^2)
+        col_step_body<T, NY, TPB, TC, U, REFRESH, LTAB, LGY>(lds, ltw, lgy, blockIdx.x, J, K, Nh, I,
+                                                             fstride, nstride, H, sI, sJ, twy_,
+                                                             ktab_, prm, n0);
+    } else {
+        col_step_body<T, NY, TPB, TC, U, REFRESH, LTAB, LGY>(lds, ltw, nullptr, blockIdx.x, J, K, Nh,
+                                                             I, fstride, nstride, H, sI, sJ, twy_,
+                                                             ktab_, prm, n0);
+    }
 }
 
 // ------------------------------------------------ single-buffer row passes

@@ -82,6 +82,14 @@ constexpr ColCfg col_cfg(int ny) {
     }
 }
 
+// The step's column pass: col_cfg, except columns of 8192, which take
+// PFC_COL_TPB8192 lanes (1024 lanes of 8 points are capped at 128 VGPRs and
+// spill; 512 lanes of 16 points have 256).
+#ifndef PFC_COL_TPB8192
+#define PFC_COL_TPB8192 512
+#endif
+constexpr ColCfg col_step_cfg(int ny) { return ny == 8192 ? ColCfg{PFC_COL_TPB8192, 1} : col_cfg(ny); }
+
 inline bool size_ok(int n) { return row_cfg<double>(n).tpb != 0 && size_built(n); }
 
 inline int ilog2(int n) {
@@ -187,7 +195,7 @@ struct Launch {
     template <int NY, int U, bool REF>
     static void col_step(pfc_ctx* c) {
         if constexpr (size_built(NY)) {
-            constexpr ColCfg g = col_cfg(NY);
+            constexpr ColCfg g = col_step_cfg(NY);
             const int H = c->Nx / 2;
             hipLaunchKernelGGL((k_col_step<T, NY, g.tpb, g.tc, U, REF>), dim3(H / g.tc + 1),
                                dim3(g.tpb * g.tc), 0, c->stream, (const C*)c->J, (const C*)c->K,
