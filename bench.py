@@ -16,7 +16,9 @@ engine's stream over the timed region; `mixed_vs_fp64_run` compares the two
 forms from the same start over the workload's 5000 steps; `cpu_baseline`
 times the CPU oracle on a bounded sample of the same workload; `batch` is the
 batch form on 96 systems of uneven sizes (2^k and 3*2^k sides from 256 to
-2048, one or three fields), beside the same systems stepped one by one.
+2048, one or three fields), beside the same systems stepped one by one;
+`other_configs` (with --other-configs) times the 2048^2 polycrystal and the
+8192^2 heterostructure in both forms.
 """
 from __future__ import annotations
 
@@ -230,6 +232,39 @@ def batch_bench(nsys, steps, warmup, seed=96):
     }
 
 
+def other_configs(steps, warmup=3):
+    """The other measured configurations at full size, both forms: the 2048^2
+    graphene polycrystal (64 seeded grains, FP64 in the reference's runs) and
+    the 8192^2 three-field heterostructure (dt = 0.25), with the inputs of
+    their GPU tests.  ms per step and grid-point-steps/s."""
+    from pfc_amd import FP64, MIXED, LATTICE_A, graphene_params, hetero_params
+    from pfc_amd.inputs import hetero_patchwork, polycrystal
+    out = {}
+    dx = LATTICE_A / 8.0
+    cases = [
+        ("graphene polycrystal 2048x2048, 1 field, dt=1",
+         lambda: (graphene_params(c=1.0, dt=1.0),
+                  polycrystal(2048, 2048, dx, dx, 64, 0.2, -0.2, LATTICE_A, sym=3, noise=0.01,
+                              boundary=2, seed=64)[None], 2048, dx)),
+        ("hetero 3-field 8192x8192, dt=0.25",
+         lambda: (hetero_params(dt=0.25, c=1.0, sigma=2.0 / 7.2552),
+                  hetero_patchwork(8192, 8192, 0.7, 0.7, n_domains=128, seed=8192, noise=0.01),
+                  8192, 0.7)),
+    ]
+    for name, make in cases:
+        params, fields, N, h = make()
+        row = {}
+        for pname, prec in (("fp64", FP64), ("mixed", MIXED)):
+            g, wall, wall_plain, ms, n = run_gpu(params, fields, N, N, h, h, prec, steps, warmup)
+            row[pname] = {"ms_per_step": 1e3 * wall_plain / steps,
+                          "grid_point_steps_per_s": N * N * steps / wall_plain,
+                          "row_pass_ms": ms[0] / max(n, 1), "column_pass_ms": ms[1] / max(n, 1)}
+            g.close()
+        out[name] = row
+        del fields
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -245,6 +280,8 @@ def main(argv=None):
     ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--batch-only", action="store_true",
                     help="run only the batched-relaxation leg and print its JSON object")
+    ap.add_argument("--other-configs", action="store_true",
+                    help="also time the 2048^2 polycrystal and the 8192^2 heterostructure")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported "
@@ -299,6 +336,8 @@ def main(argv=None):
     if a.batch_systems > 0:
         batch = batch_bench(a.batch_systems, a.batch_steps, 10)
 
+    others = other_configs(20) if a.other_configs else None
+
     cpu = None
     if a.cpu_budget > 0:
         cpu = cpu_baseline(params, fields, Nx, Ny, dx, dy, a.cpu_budget)
@@ -345,6 +384,7 @@ def main(argv=None):
                                 else "k_row_step (row pass, mixed)")},
         "cpu_baseline": cpu,
         "batch": batch,
+        "other_configs": others,
         "input_generation_s": gen_s,
     }
     print(json.dumps(out))
