@@ -1,24 +1,30 @@
 #!/usr/bin/env python3
-"""Benchmark of the pfc-relax step on one MI355X.
+"""Benchmark of the pfc-relax relaxation loop on one MI355X.
 
 Workload: 3-component graphene/hBN heterostructure PFC on a 4096 x 4096
 periodic grid (hetero.in parameters, Gaussian coupling width ~a/2, seeded
 50/50 patchwork of ~256-point domains, noise +-0.01, conserved dynamics,
-dt = 0.25).  The same K steps run once in the mixed FP32/FP64 form (the
-headline `value`) and once in FP64; each timed region starts with the fields
-resident in HBM and is bracketed by stream synchronisation.  A "step" is one
-pass of the semi-implicit update over the grid (the reference's step(),
-src/pfc-relax.c:1133-1247, including its every-100-steps refresh).
+dt = 0.25), fields resident in HBM.  A "step" is one semi-implicit update of
+the grid (the reference's step(), src/pfc-relax.c:1133-1247).  Each timed
+window is K consecutive steps run as relax() runs them: it starts at a
+multiple of 100, so it holds the t % 100 refresh steps, and it evaluates the
+free energy before every step t with t % 100 == 0 (hetero.in prints every 100
+steps).  The window is bracketed by stream synchronisation.
 
-Prints one JSON line.  `roofline` is for the dominant kernel (the longer of
-the row and column passes of the mixed step), timed with HIP events on the
-engine's stream over the timed region; `mixed_vs_fp64_run` compares the two
-forms from the same start over the workload's 5000 steps; `cpu_baseline`
-times the CPU oracle on a bounded sample of the same workload; `batch` is the
-batch form on 96 systems of uneven sizes (2^k and 3*2^k sides from 256 to
-2048, one or three fields), beside the same systems stepped one by one;
-`other_configs` (with --other-configs) times the 2048^2 polycrystal and the
-8192^2 heterostructure in both forms.
+The headline `value` is the FP64 form, the reference's precision.  The mixed
+FP32/FP64 form runs the same window beside it (`mixed`), with its distance
+from FP64 (`mixed_vs_fp64_error` after the windows, `mixed_vs_fp64_run` over
+the workload's 5000 steps).
+
+Prints one JSON line.  `roofline` is for the dominant kernel of the FP64 step
+(the longer of its row and column passes), timed with HIP events on the
+engine's stream over a second window of K steps; it also carries both passes'
+fractions and the whole step's fraction under the three-sweep byte model.
+`cpu_baseline` times the CPU oracle on a bounded sample of the same workload;
+`batch` is the batch form on 96 systems of uneven sizes (2^k and 3*2^k sides
+from 256 to 2048, one or three fields), beside the same systems stepped one
+by one; `other_configs` times the 2048^2 polycrystal and the 8192^2
+heterostructure in both forms.
 """
 from __future__ import annotations
 
@@ -35,6 +41,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0   # MI355X HBM3E
+TPRINT = 100            # hetero.in's print interval: one evaluate per 100 steps
 
 
 def col_bytes(Nx, Ny, U, esz):
@@ -55,25 +62,49 @@ def survey_bytes(Nx, Ny, U, real_size):
     return U * (16 * Nx * Ny + 96 * (Nx // 2 + 1) * Ny) * real_size / 8
 
 
-def run_gpu(params, fields, Nx, Ny, dx, dy, precision, steps, warmup):
+def relax_window(g, t0, nsteps, tprint):
+    """Steps t0 .. t0 + nsteps - 1 as relax() runs them (src/pfc-relax.c:
+    1292-1312): an evaluate (the free energy and means that print_output
+    logs) before every step t with t % tprint == 0, and each step t with
+    t % 100 == 0 refreshing the spectral state inside pfc_step.  Returns
+    the number of evaluates."""
+    t, end, nev = t0, t0 + nsteps, 0
+    while t < end:
+        if t % tprint == 0:
+            g.evaluate()
+            nev += 1
+        n = min(end, (t // tprint + 1) * tprint) - t
+        g.step(t, n)
+        t += n
+    return nev
+
+
+def run_gpu(params, fields, Nx, Ny, dx, dy, precision, steps, warmup, tprint=TPRINT):
+    """Warmup, then two windows of `steps` steps, each starting at a multiple
+    of 100 so that it holds a refresh step and an evaluate (one per tprint
+    steps, as relax() runs them).  The first window is timed by the wall clock
+    alone (the figure reported); the second records HIP events around each
+    pass on the engine's stream (per-pass times for the roofline)."""
     from pfc_amd.engine import Engine
     g = Engine(params, Nx, Ny, dx, dy, precision)
     g.set_fields(fields)
     g.step(0, warmup)
+    s0 = -(-warmup // 100) * 100
+    g.step(warmup, s0 - warmup)
     g.nan_step()                       # synchronises the stream
-    g.timing(True)
     t0 = time.perf_counter()
-    g.step(warmup, steps)
+    nev = relax_window(g, s0, steps, tprint)
     g.nan_step()
     wall = time.perf_counter() - t0
+    s1 = -(-(s0 + steps) // 100) * 100
+    g.step(s0 + steps, s1 - s0 - steps)
+    g.nan_step()
+    g.timing(True)
+    relax_window(g, s1, steps, tprint)
     ms, n = g.timing_read()
     g.timing(False)
-    # events sit between kernels; time once more without them for ms/step
-    t0 = time.perf_counter()
-    g.step(warmup + steps, steps)
-    g.nan_step()
-    wall_plain = time.perf_counter() - t0
-    return g, wall, wall_plain, ms, n
+    assert g.nan_step() == -1, "NaN in the benchmark run"
+    return g, wall, ms, n, nev, s0
 
 
 def accuracy_study(params, fields, Nx, Ny, dx, dy, nsteps, checkpoints):
@@ -136,12 +167,25 @@ def atom_positions(xd, xm, xp, dx, dy):
             "fp64_from_fp32_rounded_start": compare_atoms(ad, ap, box)}
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(params, fields, Nx, Ny, dx, dy, budget_s):
-    """Times the FP64 CPU oracle (OpenMP) on a bounded number of steps."""
+    """Times the FP64 CPU oracle (OpenMP) on a bounded number of steps, on
+    the host's CPU share: OMP_NUM_THREADS where the environment sets it (the
+    GPU box allots 16 host cores to its one GPU), else every core."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_lib import Oracle
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
-    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    os.environ["OMP_NUM_THREADS"] = str(threads)
     o = Oracle(params, Nx, Ny, dx, dy)
     o.set_fields(fields)
     o.step(1)   # warm (page in)
@@ -155,9 +199,10 @@ def cpu_baseline(params, fields, Nx, Ny, dx, dy, budget_s):
             break
     o.close()
     return {"value": Nx * Ny * n / el, "unit": "grid-point-steps/s", "cores": threads,
-            "kind": "port",
+            "kind": "port", "cpu_model": cpu_model(), "host_cpus_visible": os.cpu_count(),
             "sample": f"{n} FP64 steps of the same {Nx}x{Ny} 3-field workload on the CPU oracle "
-                      f"(oracle/pfc_oracle.c, OpenMP, {threads} threads), {el:.1f} s"}
+                      f"(oracle/pfc_oracle.c: the reference's step() restated in C, own "
+                      f"radix-4/2/3 FFT, OpenMP, {threads} threads), {el:.1f} s"}
 
 
 BATCH_SIDES = (256, 384, 512, 768, 1024, 1536, 2048)
@@ -234,9 +279,10 @@ def batch_bench(nsys, steps, warmup, seed=96):
 
 def other_configs(steps, warmup=3):
     """The other measured configurations at full size, both forms: the 2048^2
-    graphene polycrystal (64 seeded grains, FP64 in the reference's runs) and
-    the 8192^2 three-field heterostructure (dt = 0.25), with the inputs of
-    their GPU tests.  ms per step and grid-point-steps/s."""
+    graphene polycrystal (64 seeded grains, conserved, dt = 1) and the 8192^2
+    three-field heterostructure (dt = 0.25), with the inputs of their GPU
+    tests, timed like the headline (windows starting at a refresh step, one
+    evaluate per 100 steps).  ms per step and grid-point-steps/s."""
     from pfc_amd import FP64, MIXED, LATTICE_A, graphene_params, hetero_params
     from pfc_amd.inputs import hetero_patchwork, polycrystal
     out = {}
@@ -245,24 +291,36 @@ def other_configs(steps, warmup=3):
         ("graphene polycrystal 2048x2048, 1 field, dt=1",
          lambda: (graphene_params(c=1.0, dt=1.0),
                   polycrystal(2048, 2048, dx, dx, 64, 0.2, -0.2, LATTICE_A, sym=3, noise=0.01,
-                              boundary=2, seed=64)[None], 2048, dx)),
+                              boundary=2, seed=64)[None], 2048, dx, 1)),
         ("hetero 3-field 8192x8192, dt=0.25",
          lambda: (hetero_params(dt=0.25, c=1.0, sigma=2.0 / 7.2552),
                   hetero_patchwork(8192, 8192, 0.7, 0.7, n_domains=128, seed=8192, noise=0.01),
-                  8192, 0.7)),
+                  8192, 0.7, 3)),
     ]
     for name, make in cases:
-        params, fields, N, h = make()
+        params, fields, N, h, U = make()
         row = {}
         for pname, prec in (("fp64", FP64), ("mixed", MIXED)):
-            g, wall, wall_plain, ms, n = run_gpu(params, fields, N, N, h, h, prec, steps, warmup)
-            row[pname] = {"ms_per_step": 1e3 * wall_plain / steps,
-                          "grid_point_steps_per_s": N * N * steps / wall_plain,
-                          "row_pass_ms": ms[0] / max(n, 1), "column_pass_ms": ms[1] / max(n, 1)}
+            g, wall, ms, n, nev, _ = run_gpu(params, fields, N, N, h, h, prec, steps, warmup)
+            esz = 16 if prec == FP64 else 8
+            row[pname] = {"ms_per_step": 1e3 * wall / steps,
+                          "grid_point_steps_per_s": N * N * steps / wall,
+                          "row_pass_ms": ms[0] / max(n, 1), "column_pass_ms": ms[1] / max(n, 1),
+                          "step_model_frac": survey_bytes(N, N, U, esz // 2) * steps / wall / 1e9
+                          / PEAK_HBM_GBS}
             g.close()
         out[name] = row
         del fields
     return out
+
+
+def traffic_from_profile(Nx, Ny, key):
+    tf = os.path.join(ROOT, "profiles", f"traffic_step_{Nx}x{Ny}_u3.txt")
+    if os.path.exists(tf):
+        for line in open(tf):
+            if line.startswith(f"{key}:"):
+                return float(line.split(":")[1])
+    return None
 
 
 def main(argv=None):
@@ -280,19 +338,19 @@ def main(argv=None):
     ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--batch-only", action="store_true",
                     help="run only the batched-relaxation leg and print its JSON object")
-    ap.add_argument("--other-configs", action="store_true",
-                    help="also time the 2048^2 polycrystal and the 8192^2 heterostructure")
+    ap.add_argument("--other-steps", type=int, default=100,
+                    help="steps per form of the 2048^2 and 8192^2 configurations (0: skip)")
     a = ap.parse_args(argv)
     if a.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported "
-              "(the whole grid lives on one MI355X; see the sharding notes)", file=sys.stderr)
+        print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported: the whole "
+              "grid lives on one MI355X and this build has no multi-GPU path", file=sys.stderr)
         return 2
 
     if a.batch_only:
         print(json.dumps(batch_bench(a.batch_systems, a.batch_steps, 10)))
         return 0
 
-    from pfc_amd import FP64, MIXED, LATTICE_A, hetero_params
+    from pfc_amd import FP64, MIXED, hetero_params
     from pfc_amd.inputs import hetero_patchwork
 
     Nx = Ny = a.size
@@ -305,18 +363,17 @@ def main(argv=None):
     gen_s = time.perf_counter() - t0
 
     res = {}
-    for name, prec in (("mixed", MIXED), ("fp64", FP64)):
-        g, wall, wall_plain, ms, n = run_gpu(params, fields, Nx, Ny, dx, dy, prec, a.steps, a.warmup)
+    for name, prec in (("fp64", FP64), ("mixed", MIXED)):
+        g, wall, ms, n, nev, s0 = run_gpu(params, fields, Nx, Ny, dx, dy, prec, a.steps, a.warmup)
         esz = 16 if prec == FP64 else 8
         col_ms = ms[1] / max(n, 1)
         row_ms = ms[0] / max(n, 1)
         res[name] = dict(
-            wall=wall_plain, wall_evented=wall, ms_step=1e3 * wall_plain / a.steps,
-            gps=Nx * Ny * a.steps / wall_plain,
-            col_ms=col_ms, row_ms=row_ms,
+            wall=wall, ms_step=1e3 * wall / a.steps, gps=Nx * Ny * a.steps / wall,
+            col_ms=col_ms, row_ms=row_ms, evaluates=nev, window_start=s0,
             col_gbs=col_bytes(Nx, Ny, U, esz) / (col_ms * 1e-3) / 1e9,
             row_gbs=row_bytes(Nx, Ny, U, esz) / (row_ms * 1e-3) / 1e9,
-            step_model_gbs=survey_bytes(Nx, Ny, U, esz // 2) * a.steps / wall_plain / 1e9,
+            step_model_gbs=survey_bytes(Nx, Ny, U, esz // 2) * a.steps / wall / 1e9,
         )
         fe, ne = g.evaluate()
         res[name]["fave"] = fe
@@ -326,6 +383,8 @@ def main(argv=None):
     m, d = res["mixed"], res["fp64"]
     err_l2 = float(np.linalg.norm(m["fields"] - d["fields"]) / np.linalg.norm(d["fields"]))
     err_f = abs(m["fave"] - d["fave"]) / abs(d["fave"])
+    after = d["window_start"] + a.steps
+    del m["fields"], d["fields"]
 
     acc = None
     if a.accuracy_steps > 0:
@@ -336,52 +395,66 @@ def main(argv=None):
     if a.batch_systems > 0:
         batch = batch_bench(a.batch_systems, a.batch_steps, 10)
 
-    others = other_configs(20) if a.other_configs else None
+    others = other_configs(a.other_steps) if a.other_steps > 0 else None
 
     cpu = None
     if a.cpu_budget > 0:
         cpu = cpu_baseline(params, fields, Nx, Ny, dx, dy, a.cpu_budget)
 
-    # the roofline is reported for the pass that takes longer in the mixed step
-    dom = "column" if m["col_ms"] >= m["row_ms"] else "row"
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", f"traffic_step_{Nx}x{Ny}_u3.txt")
-    if os.path.exists(tf):
-        for line in open(tf):
-            if line.startswith(f"mixed_{dom}_bytes_per_launch:"):
-                traffic = float(line.split(":")[1])
-    achieved = m["col_gbs"] if dom == "column" else m["row_gbs"]
+    def kernel_roof(r, pname, esz):
+        dom = "column" if r["col_ms"] >= r["row_ms"] else "row"
+        ach = r["col_gbs"] if dom == "column" else r["row_gbs"]
+        return {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": ach / PEAK_HBM_GBS,
+                "traffic": traffic_from_profile(Nx, Ny, f"{pname}_{dom}_bytes_per_launch"),
+                "kernel": ("k_col_step (column pass, " if dom == "column" else
+                           "k_row_step (row pass, ") + pname + ")",
+                "algorithmic_bytes_per_launch": (col_bytes if dom == "column" else row_bytes)(
+                    Nx, Ny, U, esz),
+                "avg_launch_ms": r["col_ms"] if dom == "column" else r["row_ms"],
+                "row_pass_frac": r["row_gbs"] / PEAK_HBM_GBS,
+                "column_pass_frac": r["col_gbs"] / PEAK_HBM_GBS,
+                "step_model_frac": r["step_model_gbs"] / PEAK_HBM_GBS}
 
+    roof = kernel_roof(d, "fp64", 16)
+    roof["step_model_note"] = ("step_model_frac: the three-sweep model's bytes per step, "
+                               "U (16 NxNy + 96 (Nx/2+1) Ny) B in FP64, times the steps, over "
+                               "the whole timed wall clock (evaluates and refreshes included)")
     out = {
         "metric": "grid-point-steps/s at 4096² 3-field, mixed vs FP64 on 1 GPU; HBM GB/s",
-        "value": m["gps"],
+        "value": d["gps"],
         "unit": "grid-point-steps/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": m["ms_step"],
+        "ms_per_step": d["ms_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 (f64 reductions, k=0 mode, box)",
+        "dtype": "f64",
         "data": "synthetic: seeded graphene/hBN Voronoi patchwork (hetero.in parameters)",
-        "config": {"workload": f"hetero 3-field PFC {Nx}x{Ny} conserved dt=0.25, mixed precision",
-                   "grid": [Nx, Ny], "fields": U, "parallelism": "1 GPU"},
+        "config": {"workload": f"hetero 3-field PFC {Nx}x{Ny} conserved dt=0.25, FP64 (the "
+                               f"reference's precision); the mixed FP32/FP64 form beside it",
+                   "grid": [Nx, Ny], "fields": U, "parallelism": "1 GPU",
+                   "timed_window": f"steps {d['window_start']}..{d['window_start'] + a.steps - 1} "
+                                   f"with {d['evaluates']} evaluate(s) (one per {TPRINT} steps, "
+                                   f"as relax() prints) and the t % 100 refresh steps"},
         "fp64": {"value": d["gps"], "ms_per_step": d["ms_step"],
                  "column_pass_GBps": d["col_gbs"], "row_pass_GBps": d["row_gbs"],
                  "column_pass_ms": d["col_ms"], "row_pass_ms": d["row_ms"],
                  "step_model_GBps": d["step_model_gbs"]},
-        "mixed": {"column_pass_GBps": m["col_gbs"], "row_pass_GBps": m["row_gbs"],
+        "mixed": {"value": m["gps"], "ms_per_step": m["ms_step"],
+                  "column_pass_GBps": m["col_gbs"], "row_pass_GBps": m["row_gbs"],
                   "column_pass_ms": m["col_ms"], "row_pass_ms": m["row_ms"],
-                  "step_model_GBps": m["step_model_gbs"]},
+                  "step_model_GBps": m["step_model_gbs"],
+                  "roofline": kernel_roof(m, "mixed", 8),
+                  "accuracy_note": "energy per area against FP64 is within 1e-6 over the first "
+                                   "~100 steps of this random start only; see mixed_vs_fp64_run"},
         "speedup_mixed_vs_fp64": m["gps"] / d["gps"],
         "mixed_vs_fp64_error": {"density_rel_l2": err_l2, "energy_per_area_rel": err_f,
-                                "after_steps": a.warmup + 2 * a.steps},
+                                "after_steps": after},
         "mixed_vs_fp64_run": acc,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
-                     "kernel": ("k_col_step (column pass, mixed)" if dom == "column"
-                                else "k_row_step (row pass, mixed)")},
+        "roofline": roof,
         "cpu_baseline": cpu,
         "batch": batch,
         "other_configs": others,

@@ -22,6 +22,17 @@
  */
 #include "pfc_oracle.h"
 
+/* ORACLE_MUTANT plants one known mistake, for tests/test_oracle_mutants.py,
+ * which builds such copies and checks that the closed-form pins reject them:
+ *   1  step: m_u and m_v swapped in the gamma_uv term of mc[v] (:1196-1197)
+ *   2  relax: the step taken before the print and write of the same t
+ *   3  relax: the print taken before the optimisation of the same t
+ *   4  relax: the snapshot taken after the step of the same t
+ * The shipped build (the Makefile's liboracle.so) has ORACLE_MUTANT 0. */
+#ifndef ORACLE_MUTANT
+#define ORACLE_MUTANT 0
+#endif
+
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -416,7 +427,11 @@ int oracle_step(oracle* o, int t) {
             for (int v = u + 1; v < U; v++) {
                 const double mu = o->m[u][ij], mv = o->m[v][ij];
                 mc[u] += p->alpha[u][v] * mv + 0.5 * p->gamma[u][v] * (2.0 * mu * mv + mv * mv);
+#if ORACLE_MUTANT == 1
+                mc[v] += p->alpha[u][v] * mu + 0.5 * p->gamma[u][v] * (2.0 * mu * mv + mv * mv);
+#else
                 mc[v] += p->alpha[u][v] * mu + 0.5 * p->gamma[u][v] * (2.0 * mv * mu + mu * mu);
+#endif
             }
         }
         for (int u = 0; u < U; u++) {
@@ -630,7 +645,12 @@ int oracle_relax(oracle* o, int tend, int Tprint, int Topt, int xopt, int yopt,
     int rows = 0, ns = 0;
     double d = 0.0001;
     for (int t = 0; t <= tend; t++) {
+#if ORACLE_MUTANT == 2
+        if (t < tend && oracle_step(o, t) == 39) return -39;
+#endif
+#if ORACLE_MUTANT != 3
         if (Topt > 0 && t > 0 && t % Topt == 0) oracle_optimize(o, &d, xopt, yopt);
+#endif
         if (Tprint > 0 && t % Tprint == 0) {
             double fave, nave[PFC_MAX_FIELDS];
             oracle_evaluate(o, &fave, nave);
@@ -641,13 +661,21 @@ int oracle_relax(oracle* o, int tend, int Tprint, int Topt, int xopt, int yopt,
                 rows++;
             }
         }
+#if ORACLE_MUTANT == 3
+        if (Topt > 0 && t > 0 && t % Topt == 0) oracle_optimize(o, &d, xopt, yopt);
+#endif
+#if ORACLE_MUTANT == 4
+        if (t < tend && oracle_step(o, t) == 39) return -39;
+#endif
         if (ns < n_snap && t == snap_steps[ns]) {
             oracle_get_fields(o, snaps + (size_t)ns * U * o->nr);
             ns++;
         }
+#if ORACLE_MUTANT != 2 && ORACLE_MUTANT != 4
         if (t < tend) {
             if (oracle_step(o, t) == 39) return -39;
         }
+#endif
     }
     return rows;
 }

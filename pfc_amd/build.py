@@ -46,7 +46,18 @@ def _run(cmd):
     return out
 
 
-def build_lib(verbose=False, libdir=None, objdir=None, defines=()):
+def _stale(obj, src):
+    """True unless obj is newer than src, every header under csrc/ and the
+    public header (the units include each other's headers freely)."""
+    if not os.path.exists(obj):
+        return True
+    deps = [src, os.path.join(ROOT, "include", "pfc_relax.h")]
+    deps += [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hpp", ".h"))]
+    t = os.path.getmtime(obj)
+    return any(os.path.getmtime(d) > t for d in deps)
+
+
+def build_lib(verbose=False, libdir=None, objdir=None, defines=(), incremental=False):
     libdir = libdir or LIBDIR
     objdir = objdir or OBJDIR
     os.makedirs(objdir, exist_ok=True)
@@ -56,9 +67,11 @@ def build_lib(verbose=False, libdir=None, objdir=None, defines=()):
     for u in HIP_UNITS:
         obj = os.path.join(objdir, u.replace(".hip", ".o"))
         objs.append(obj)
+        if incremental and not defines and not _stale(obj, os.path.join(CSRC, u)):
+            continue
         cmds.append([HIPCC, *HIP_FLAGS, *[f"-D{d}" for d in defines], "-c", os.path.join(CSRC, u),
                      "-o", obj])
-    with cf.ThreadPoolExecutor(max_workers=len(cmds)) as ex:
+    with cf.ThreadPoolExecutor(max_workers=max(1, len(cmds))) as ex:
         for out in ex.map(_run, cmds):
             if verbose and out.strip():
                 print(out)
@@ -124,8 +137,10 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--lib", action="store_true", help="only the HIP library")
     ap.add_argument("-v", "--verbose", action="store_true")
+    ap.add_argument("--incremental", action="store_true",
+                    help="recompile only the HIP units older than their sources or headers")
     a = ap.parse_args(argv)
-    lib = build_lib(a.verbose)
+    lib = build_lib(a.verbose, incremental=a.incremental)
     print("built", lib)
     if not a.lib:
         print("built", build_host())

@@ -145,10 +145,12 @@ int ensure_events(pfc_ctx* c, int n) {
     while (cap < 3 * n) cap *= 2;
     hipEvent_t* ev = (hipEvent_t*)realloc(c->ev, sizeof(hipEvent_t) * cap);
     if (!ev) return PFC_ENOMEM;
-    for (int i = c->nev; i < cap; i++)
-        if (hipEventCreate(&ev[i]) != hipSuccess) return PFC_EDEVICE;
-    c->ev = ev;
-    c->nev = cap;
+    c->ev = ev;   // the old block may have moved; keep the new one at once
+    // nev counts created events only, so that free_all destroys valid handles
+    while (c->nev < cap) {
+        if (hipEventCreate(&c->ev[c->nev]) != hipSuccess) return PFC_EDEVICE;
+        c->nev++;
+    }
     return PFC_OK;
 }
 
@@ -185,7 +187,9 @@ const char* pfc_strerror(int code) {
         case PFC_EINVAL: return "invalid argument";
         case PFC_EDEVICE: return "HIP runtime error";
         case PFC_ENOMEM: return "device allocation failed";
-        case PFC_ESIZE: return "unsupported grid size (Nx and Ny must be powers of two in [64, 8192])";
+        case PFC_ESIZE:
+            return "grid size not supported by this build (Nx and Ny: 2^k in [64, 8192], or 384, 768, "
+                   "1536)";
         case PFC_ESTATE: return "call out of order";
         default: return "unknown error";
     }

@@ -1,0 +1,259 @@
+// pfc_launch_impl.hpp -- definitions of the launchers declared in
+// pfc_launch.hpp: one switch over the compiled sizes per kernel family.
+// Included only by the per-precision instantiation units.
+#pragma once
+#include "pfc_launch.hpp"
+
+namespace pfc {
+
+// Launchers for one element precision T; explicitly instantiated for double
+// and float in their own translation units.
+template <typename T>
+struct LaunchImpl {
+    using C = cpx<T>;
+
+    template <int NX, int U, bool REF>
+    static void row_step(pfc_ctx* c, int t) {
+        if constexpr (size_built(NX)) {
+            constexpr RowCfg g = row_cfg<T>(NX);
+            hipLaunchKernelGGL((k_row_step<T, NX, g.tpb, g.tr, U, REF>), dim3(c->Ny / g.tr),
+                               dim3(g.tpb * g.tr), 0, c->stream, (const C*)c->I, (C*)c->J, (C*)c->K,
+                               c->nk, c->sI, c->sJ, (const C*)c->twx, c->dprm, c->nan_flag, t);
+        }
+    }
+    template <int NX, int U>
+    static void row_step_u(pfc_ctx* c, int t, bool ref) {
+        if (ref) row_step<NX, U, true>(c, t);
+        else row_step<NX, U, false>(c, t);
+    }
+    template <int NX>
+    static void row_step_n(pfc_ctx* c, int t, bool ref) {
+        switch (c->U) {
+            case 1: row_step_u<NX, 1>(c, t, ref); break;
+            case 2: row_step_u<NX, 2>(c, t, ref); break;
+            case 3: row_step_u<NX, 3>(c, t, ref); break;
+            case 4: row_step_u<NX, 4>(c, t, ref); break;
+        }
+    }
+    static void row_step_any(pfc_ctx* c, int t, bool ref) {
+        switch (c->Nx) {
+            case 64: row_step_n<64>(c, t, ref); break;
+            case 128: row_step_n<128>(c, t, ref); break;
+            case 256: row_step_n<256>(c, t, ref); break;
+            case 384: row_step_n<384>(c, t, ref); break;
+            case 512: row_step_n<512>(c, t, ref); break;
+            case 768: row_step_n<768>(c, t, ref); break;
+            case 1024: row_step_n<1024>(c, t, ref); break;
+            case 1536: row_step_n<1536>(c, t, ref); break;
+            case 2048: row_step_n<2048>(c, t, ref); break;
+            case 4096: row_step_n<4096>(c, t, ref); break;
+            case 8192: row_step_n<8192>(c, t, ref); break;
+        }
+    }
+
+    template <int NY, int U, bool REF>
+    static void col_step(pfc_ctx* c) {
+        if constexpr (size_built(NY)) {
+            constexpr ColCfg g = col_step_cfg(NY);
+            const int H = c->Nx / 2;
+            hipLaunchKernelGGL((k_col_step<T, NY, g.tpb, g.tc, U, REF>), dim3(H / g.tc + 1),
+                               dim3(g.tpb * g.tc), 0, c->stream, (const C*)c->J, (const C*)c->K,
+                               (C*)c->Nh, (C*)c->I, c->nk, c->nkn, H, c->sI, c->sJ, (const C*)c->twy,
+                               (const T*)c->ktab, c->dprm, c->n0);
+        }
+    }
+    template <int NY, int U>
+    static void col_step_u(pfc_ctx* c, bool ref) {
+        if (ref) col_step<NY, U, true>(c);
+        else col_step<NY, U, false>(c);
+    }
+    template <int NY>
+    static void col_step_n(pfc_ctx* c, bool ref) {
+        switch (c->U) {
+            case 1: col_step_u<NY, 1>(c, ref); break;
+            case 2: col_step_u<NY, 2>(c, ref); break;
+            case 3: col_step_u<NY, 3>(c, ref); break;
+            case 4: col_step_u<NY, 4>(c, ref); break;
+        }
+    }
+    static void col_step_any(pfc_ctx* c, bool ref) {
+        switch (c->Ny) {
+            case 64: col_step_n<64>(c, ref); break;
+            case 128: col_step_n<128>(c, ref); break;
+            case 256: col_step_n<256>(c, ref); break;
+            case 384: col_step_n<384>(c, ref); break;
+            case 512: col_step_n<512>(c, ref); break;
+            case 768: col_step_n<768>(c, ref); break;
+            case 1024: col_step_n<1024>(c, ref); break;
+            case 1536: col_step_n<1536>(c, ref); break;
+            case 2048: col_step_n<2048>(c, ref); break;
+            case 4096: col_step_n<4096>(c, ref); break;
+            case 8192: col_step_n<8192>(c, ref); break;
+        }
+    }
+
+    template <int NX>
+    static void row_c2r(pfc_ctx* c, const void* src, int sS, void* dst, void* K, int nbuf) {
+        if constexpr (size_built(NX)) {
+            constexpr RowCfg g = row_cfg<T>(NX);
+            hipLaunchKernelGGL((k_row_c2r<T, NX, g.tpb, g.tr>), dim3(c->Ny / g.tr, nbuf),
+                               dim3(g.tpb * g.tr), 0, c->stream, (const C*)src, (T*)dst, (C*)K, c->nk,
+                               c->nr, sS, c->sJ, (const C*)c->twx);
+        }
+    }
+    static void row_c2r_any(pfc_ctx* c, const void* src, int sS, void* dst, void* K, int nbuf) {
+        switch (c->Nx) {
+            case 64: row_c2r<64>(c, src, sS, dst, K, nbuf); break;
+            case 128: row_c2r<128>(c, src, sS, dst, K, nbuf); break;
+            case 256: row_c2r<256>(c, src, sS, dst, K, nbuf); break;
+            case 384: row_c2r<384>(c, src, sS, dst, K, nbuf); break;
+            case 512: row_c2r<512>(c, src, sS, dst, K, nbuf); break;
+            case 768: row_c2r<768>(c, src, sS, dst, K, nbuf); break;
+            case 1024: row_c2r<1024>(c, src, sS, dst, K, nbuf); break;
+            case 1536: row_c2r<1536>(c, src, sS, dst, K, nbuf); break;
+            case 2048: row_c2r<2048>(c, src, sS, dst, K, nbuf); break;
+            case 4096: row_c2r<4096>(c, src, sS, dst, K, nbuf); break;
+            case 8192: row_c2r<8192>(c, src, sS, dst, K, nbuf); break;
+        }
+    }
+    template <int NX>
+    static void row_r2c(pfc_ctx* c, const void* src, void* dst, int sD, int nbuf) {
+        if constexpr (size_built(NX)) {
+            constexpr RowCfg g = row_cfg<T>(NX);
+            hipLaunchKernelGGL((k_row_r2c<T, NX, g.tpb, g.tr>), dim3(c->Ny / g.tr, nbuf),
+                               dim3(g.tpb * g.tr), 0, c->stream, (const T*)src, (C*)dst, c->nk, c->nr,
+                               sD, (const C*)c->twx, (T)(1.0 / NX));
+        }
+    }
+    static void row_r2c_any(pfc_ctx* c, const void* src, void* dst, int sD, int nbuf) {
+        switch (c->Nx) {
+            case 64: row_r2c<64>(c, src, dst, sD, nbuf); break;
+            case 128: row_r2c<128>(c, src, dst, sD, nbuf); break;
+            case 256: row_r2c<256>(c, src, dst, sD, nbuf); break;
+            case 384: row_r2c<384>(c, src, dst, sD, nbuf); break;
+            case 512: row_r2c<512>(c, src, dst, sD, nbuf); break;
+            case 768: row_r2c<768>(c, src, dst, sD, nbuf); break;
+            case 1024: row_r2c<1024>(c, src, dst, sD, nbuf); break;
+            case 1536: row_r2c<1536>(c, src, dst, sD, nbuf); break;
+            case 2048: row_r2c<2048>(c, src, dst, sD, nbuf); break;
+            case 4096: row_r2c<4096>(c, src, dst, sD, nbuf); break;
+            case 8192: row_r2c<8192>(c, src, dst, sD, nbuf); break;
+        }
+    }
+    template <int NY>
+    static void col_fwd(pfc_ctx* c, const void* src, int sS, void* dst, int nbuf, double scale) {
+        if constexpr (size_built(NY)) {
+            constexpr ColCfg g = col_cfg(NY);
+            const int H = c->Nx / 2;
+            hipLaunchKernelGGL((k_col_fwd<T, NY, g.tpb, g.tc>), dim3(H / g.tc + 1, nbuf),
+                               dim3(g.tpb * g.tc), 0, c->stream, (const C*)src, (C*)dst, c->nk,
+                               c->nkn, H, sS, (const C*)c->twy, scale);
+        }
+    }
+    static void col_fwd_any(pfc_ctx* c, const void* src, int sS, void* dst, int nbuf, double scale) {
+        switch (c->Ny) {
+            case 64: col_fwd<64>(c, src, sS, dst, nbuf, scale); break;
+            case 128: col_fwd<128>(c, src, sS, dst, nbuf, scale); break;
+            case 256: col_fwd<256>(c, src, sS, dst, nbuf, scale); break;
+            case 384: col_fwd<384>(c, src, sS, dst, nbuf, scale); break;
+            case 512: col_fwd<512>(c, src, sS, dst, nbuf, scale); break;
+            case 768: col_fwd<768>(c, src, sS, dst, nbuf, scale); break;
+            case 1024: col_fwd<1024>(c, src, sS, dst, nbuf, scale); break;
+            case 1536: col_fwd<1536>(c, src, sS, dst, nbuf, scale); break;
+            case 2048: col_fwd<2048>(c, src, sS, dst, nbuf, scale); break;
+            case 4096: col_fwd<4096>(c, src, sS, dst, nbuf, scale); break;
+            case 8192: col_fwd<8192>(c, src, sS, dst, nbuf, scale); break;
+        }
+    }
+    template <int NY>
+    static void col_smooth(pfc_ctx* c, const void* src, void* dst, double sigma) {
+        if constexpr (size_built(NY)) {
+            constexpr ColCfg g = col_cfg(NY);
+            const int H = c->Nx / 2;
+            hipLaunchKernelGGL((k_col_smooth<T, NY, g.tpb, g.tc>), dim3(H / g.tc + 1),
+                               dim3(g.tpb * g.tc), 0, c->stream, (const C*)src, (C*)dst, H, c->sC,
+                               (const C*)c->twy, c->hp.dkx, c->hp.dky, 0.5 / sigma / sigma);
+        }
+    }
+    static void col_smooth_any(pfc_ctx* c, const void* src, void* dst, double sigma) {
+        switch (c->Ny) {
+            case 64: col_smooth<64>(c, src, dst, sigma); break;
+            case 128: col_smooth<128>(c, src, dst, sigma); break;
+            case 256: col_smooth<256>(c, src, dst, sigma); break;
+            case 384: col_smooth<384>(c, src, dst, sigma); break;
+            case 512: col_smooth<512>(c, src, dst, sigma); break;
+            case 768: col_smooth<768>(c, src, dst, sigma); break;
+            case 1024: col_smooth<1024>(c, src, dst, sigma); break;
+            case 1536: col_smooth<1536>(c, src, dst, sigma); break;
+            case 2048: col_smooth<2048>(c, src, dst, sigma); break;
+            case 4096: col_smooth<4096>(c, src, dst, sigma); break;
+            case 8192: col_smooth<8192>(c, src, dst, sigma); break;
+        }
+    }
+    template <int NY>
+    static void col_eval(pfc_ctx* c) {
+        if constexpr (size_built(NY)) {
+            constexpr ColCfg g = col_cfg(NY);
+            const int H = c->Nx / 2;
+            hipLaunchKernelGGL((k_col_eval<T, NY, g.tpb, g.tc>), dim3(H / g.tc + 1, c->U),
+                               dim3(g.tpb * g.tc), 0, c->stream, (const C*)c->K, (C*)c->Nh, (C*)c->O,
+                               c->nk, c->nkn, H, c->sJ, (const C*)c->twy, c->dprm);
+        }
+    }
+    static void col_eval_any(pfc_ctx* c) {
+        switch (c->Ny) {
+            case 64: col_eval<64>(c); break;
+            case 128: col_eval<128>(c); break;
+            case 256: col_eval<256>(c); break;
+            case 384: col_eval<384>(c); break;
+            case 512: col_eval<512>(c); break;
+            case 768: col_eval<768>(c); break;
+            case 1024: col_eval<1024>(c); break;
+            case 1536: col_eval<1536>(c); break;
+            case 2048: col_eval<2048>(c); break;
+            case 4096: col_eval<4096>(c); break;
+            case 8192: col_eval<8192>(c); break;
+        }
+    }
+    template <int U>
+    static void energy(pfc_ctx* c) {
+        hipLaunchKernelGGL((k_energy<T, U, kEnergyBS>), dim3(kEnergyBlocks), dim3(kEnergyBS), 0,
+                           c->stream, (const T*)c->mreal, (const T*)c->R, c->f, c->nr, c->partials,
+                           c->dprm);
+    }
+    static void energy_any(pfc_ctx* c) {
+        switch (c->U) {
+            case 1: energy<1>(c); break;
+            case 2: energy<2>(c); break;
+            case 3: energy<3>(c); break;
+            case 4: energy<4>(c); break;
+        }
+    }
+};
+
+template <typename T>
+void Launch<T>::row_step_any(pfc_ctx* c, int t, bool ref) { LaunchImpl<T>::row_step_any(c, t, ref); }
+template <typename T>
+void Launch<T>::col_step_any(pfc_ctx* c, bool ref) { LaunchImpl<T>::col_step_any(c, ref); }
+template <typename T>
+void Launch<T>::row_c2r_any(pfc_ctx* c, const void* src, int sS, void* dst, void* K, int nbuf) {
+    LaunchImpl<T>::row_c2r_any(c, src, sS, dst, K, nbuf);
+}
+template <typename T>
+void Launch<T>::row_r2c_any(pfc_ctx* c, const void* src, void* dst, int sD, int nbuf) {
+    LaunchImpl<T>::row_r2c_any(c, src, dst, sD, nbuf);
+}
+template <typename T>
+void Launch<T>::col_fwd_any(pfc_ctx* c, const void* src, int sS, void* dst, int nbuf, double scale) {
+    LaunchImpl<T>::col_fwd_any(c, src, sS, dst, nbuf, scale);
+}
+template <typename T>
+void Launch<T>::col_smooth_any(pfc_ctx* c, const void* src, void* dst, double sigma) {
+    LaunchImpl<T>::col_smooth_any(c, src, dst, sigma);
+}
+template <typename T>
+void Launch<T>::col_eval_any(pfc_ctx* c) { LaunchImpl<T>::col_eval_any(c); }
+template <typename T>
+void Launch<T>::energy_any(pfc_ctx* c) { LaunchImpl<T>::energy_any(c); }
+
+}  // namespace pfc

@@ -17,15 +17,18 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(ORACLE_SO):
+def lib(path=None):
+    """The oracle library at path (default: oracle/liboracle.so, built on
+    first use).  Other paths are builds of the same source, such as the
+    seeded-mistake builds of tests/test_oracle_mutants.py."""
+    path = path or ORACLE_SO
+    if path not in _libs:
+        if path == ORACLE_SO and not os.path.exists(ORACLE_SO):
             subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
-        L = ctypes.CDLL(ORACLE_SO)
+        L = ctypes.CDLL(path)
         dp = ctypes.POINTER(ctypes.c_double)
         vp = ctypes.c_void_p
         L.oracle_fft1d.argtypes = [ctypes.c_int, dp, dp, ctypes.c_int]
@@ -50,8 +53,8 @@ def lib():
         L.oracle_relax.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                    ctypes.c_int, dp, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                    ctypes.c_int, dp]
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return _libs[path]
 
 
 def _dp(a):
@@ -85,18 +88,19 @@ def c2r2d(X, Nx):
 class Oracle:
     """One oracle solver instance (the reference's struct S on one rank)."""
 
-    def __init__(self, params: PfcParams, Nx: int, Ny: int, dx: float, dy: float):
+    def __init__(self, params: PfcParams, Nx: int, Ny: int, dx: float, dy: float, so=None):
+        self.L = lib(so)
         self.p = params
         self.U = params.U
         self.Nx, self.Ny = Nx, Ny
-        self._h = lib().oracle_create(ctypes.byref(params), Nx, Ny)
+        self._h = self.L.oracle_create(ctypes.byref(params), Nx, Ny)
         if not self._h:
             raise ValueError("oracle_create failed")
-        lib().oracle_set_box(self._h, dx, dy)
+        self.L.oracle_set_box(self._h, dx, dy)
 
     def close(self):
         if self._h:
-            lib().oracle_destroy(self._h)
+            self.L.oracle_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -104,41 +108,41 @@ class Oracle:
 
     def set_fields(self, f):
         f = np.ascontiguousarray(f, dtype=np.float64).reshape(self.U, self.Ny, self.Nx)
-        lib().oracle_set_fields(self._h, _dp(f))
+        self.L.oracle_set_fields(self._h, _dp(f))
 
     def get_fields(self):
         out = np.empty((self.U, self.Ny, self.Nx))
-        lib().oracle_get_fields(self._h, _dp(out))
+        self.L.oracle_get_fields(self._h, _dp(out))
         return out
 
     def get_energy(self):
         out = np.empty((self.Ny, self.Nx))
-        lib().oracle_get_energy(self._h, _dp(out))
+        self.L.oracle_get_energy(self._h, _dp(out))
         return out
 
     def spectral(self, u):
         out = np.empty((self.Nx // 2 + 1, self.Ny), dtype=np.complex128)
-        lib().oracle_get_spectral(self._h, u, _dp(out))
+        self.L.oracle_get_spectral(self._h, u, _dp(out))
         return out
 
     def AB(self, u):
         nk = (self.Nx // 2 + 1) * self.Ny
         A = np.empty(nk)
         B = np.empty(nk)
-        lib().oracle_get_AB(self._h, u, _dp(A), _dp(B))
+        self.L.oracle_get_AB(self._h, u, _dp(A), _dp(B))
         return A.reshape(self.Nx // 2 + 1, self.Ny), B.reshape(self.Nx // 2 + 1, self.Ny)
 
     def set_box(self, dx, dy):
-        lib().oracle_set_box(self._h, dx, dy)
+        self.L.oracle_set_box(self._h, dx, dy)
 
     def box(self):
         dx = ctypes.c_double()
         dy = ctypes.c_double()
-        lib().oracle_get_box(self._h, ctypes.byref(dx), ctypes.byref(dy))
+        self.L.oracle_get_box(self._h, ctypes.byref(dx), ctypes.byref(dy))
         return dx.value, dy.value
 
     def step(self, t):
-        return lib().oracle_step(self._h, t)
+        return self.L.oracle_step(self._h, t)
 
     def steps(self, t0, n):
         for t in range(t0, t0 + n):
@@ -150,12 +154,12 @@ class Oracle:
     def evaluate(self):
         fave = ctypes.c_double()
         nave = np.zeros(self.U)
-        lib().oracle_evaluate(self._h, ctypes.byref(fave), _dp(nave))
+        self.L.oracle_evaluate(self._h, ctypes.byref(fave), _dp(nave))
         return fave.value, nave
 
     def optimize(self, d, xopt=1, yopt=1):
         dd = ctypes.c_double(d)
-        lib().oracle_optimize(self._h, ctypes.byref(dd), xopt, yopt)
+        self.L.oracle_optimize(self._h, ctypes.byref(dd), xopt, yopt)
         return dd.value
 
     def relax(self, tend, Tprint, Topt=0, xopt=1, yopt=1, snap_steps=()):
@@ -163,7 +167,7 @@ class Oracle:
         log = np.zeros((rows, 4 + self.U))
         snaps = np.zeros((max(len(snap_steps), 1), self.U, self.Ny, self.Nx))
         ss = (ctypes.c_int * max(len(snap_steps), 1))(*snap_steps)
-        n = lib().oracle_relax(self._h, tend, Tprint, Topt, xopt, yopt, _dp(log), rows, ss,
+        n = self.L.oracle_relax(self._h, tend, Tprint, Topt, xopt, yopt, _dp(log), rows, ss,
                                len(snap_steps), _dp(snaps))
         if n < 0:
             raise FloatingPointError(f"oracle relax failed with code {-n}")

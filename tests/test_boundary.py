@@ -37,7 +37,7 @@ def test_library_exports_every_header_symbol():
 
 def test_strerror_and_create_without_device():
     lib = engine.load()
-    assert "unsupported grid size" in engine.strerror(-4)
+    assert "grid size not supported by this build" in engine.strerror(-4)
     from pfc_amd import graphene_params
     p = graphene_params()
     h = ctypes.c_void_p()

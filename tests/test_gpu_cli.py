@@ -40,13 +40,19 @@ def run_cli(cwd, inp, env=None):
     return r.returncode, r.stdout, r.stderr
 
 
-def parse_log(text, U):
+def parse_log(text, U, pprint):
+    """Rows [t, dx, dy, fave, nave...] of a log; each line must be byte for
+    byte print_output's "%d %d %.*lf %.*lf %.*lf " + U x "%.*lf "
+    (src/pfc-relax.c:880-884, trailing space included) at pprint decimals,
+    column 2 (elapsed whole seconds) aside."""
     rows = []
-    for line in text.strip().splitlines():
-        v = line.split()
-        assert len(v) == 5 + U, line
-        assert line.endswith(" ") or True
-        rows.append([float(v[0]), float(v[2]), float(v[3]), float(v[4])] + [float(x) for x in v[5:]])
+    for line in text.splitlines():
+        v = line.split(" ")
+        assert len(v) == 6 + U and v[-1] == "", repr(line)
+        t, el = int(v[0]), int(v[1])
+        vals = [float(x) for x in v[2:-1]]
+        assert line == f"{t} {el} " + "".join(f"{x:.{pprint}f} " for x in vals), repr(line)
+        rows.append([float(t)] + vals)
     return np.array(rows)
 
 
@@ -83,30 +89,36 @@ def test_graphene_run_matches_oracle(gpu, tmp_path, c, topt):
     (tmp_path / "run.in").write_text(HONEY.format(c=c, dx=repr(dx), dy=repr(dy), topt=topt))
     rc, out, err = run_cli(tmp_path, "run.in")
     assert rc == 0, err
-    log = parse_log(out, 1)
+    log = parse_log(out, 1, 14)
     assert (tmp_path / "run.out").read_text() == out
     o = Oracle(graphene_params(c=float(c)), Nx, Ny, dx, dy)
     o.set_fields(f[None])
     olog, snaps = o.relax(250, 100, Topt=topt, xopt=1, yopt=1, snap_steps=(0, 100, 250))
     assert np.array_equal(log[:, 0], olog[:, 0])
-    # dx, dy: with topt = 50 the box comes out of five chained searches, each
-    # moving it by ~1e-12 relative per unit of energy round-off (see below),
-    # and the GPU's FMA contraction rounds differently from the oracle's
-    # unfused arithmetic (measured up to 1.9e-11)
-    np.testing.assert_allclose(log[:, 1:3], olog[:, 1:3], rtol=5e-11, atol=0)
-    # fave depends on the box and follows it; nave does not
-    np.testing.assert_allclose(log[:, 3], olog[:, 3], rtol=5e-11, atol=1e-14)   # fave
-    np.testing.assert_allclose(log[:, 4], olog[:, 4], rtol=1e-11, atol=1e-14)   # nave
-    # Each box search fits a parabola through energies whose second difference
-    # is ~1e-9 of the energy, so round-off-level energy differences move the
-    # chosen box by ~1e-12 relative; the trajectory after a search carries that.
-    tol = 1e-10 if topt == 0 else 1e-9
+    box = float(np.max(np.abs(log[:, 1:3] - olog[:, 1:3]) / olog[:, 1:3]))
+    fav = float(np.max(np.abs(log[:, 3] - olog[:, 3]) / np.abs(olog[:, 3])))
+    nav = float(np.max(np.abs(log[:, 4] - olog[:, 4])))
+    errs = []
     for k, t in enumerate((0, 100, 250)):
         hdr, n = read_density(tmp_path / f"out-{t}.n")
         assert hdr[:2] == [str(Nx), str(Ny)]
-        err_l2 = np.linalg.norm(n - snaps[k, 0]) / np.linalg.norm(snaps[k, 0])
-        assert err_l2 < tol, (t, err_l2)
+        errs.append(float(np.linalg.norm(n - snaps[k, 0]) / np.linalg.norm(snaps[k, 0])))
         assert (tmp_path / f"out-{t}.f").exists()
+    print(f"c={c} topt={topt}: box {box:.2e} fave {fav:.2e} nave {nav:.2e} density {errs}")
+    if topt == 0:
+        # no box search: the north star's bars (energy 1e-12, density 1e-10)
+        assert box == 0.0 and fav < 1e-12 and nav < 1e-14
+        assert max(errs) < 1e-10, errs
+    else:
+        # Each box search fits a parabola through energies whose second
+        # difference is ~1e-9 of the energy, so a round-off difference in the
+        # energies moves the box by ~1e-12 relative, and later steps carry
+        # it.  Two CPU builds of the oracle that differ only in FMA
+        # contraction already differ by 3e-12 (box) and 1.2e-12 (fave) here
+        # (tests/test_roundoff_amplification.py); the GPU's transforms round
+        # differently again.  The bars are set above that spread.
+        assert box < 5e-11 and fav < 5e-11 and nav < 1e-14
+        assert max(errs) < 1e-9, errs
 
 
 HETERO = """I\ta.n\tb.n\tc.n
@@ -155,7 +167,7 @@ def test_hetero_run(gpu, tmp_path, mixed):
     (tmp_path / "h.in").write_text(HETERO)
     rc, out, err = run_cli(tmp_path, "h.in", env={"PFC_PRECISION": "mixed"} if mixed else None)
     assert rc == 0, err
-    log = parse_log(out, 3)
+    log = parse_log(out, 3, 12)
     o = Oracle(hetero_params(), Nx, Ny, 0.7, 0.7)
     o.set_fields(f)
     olog, snaps = o.relax(100, 50, snap_steps=(0, 50, 100))
@@ -186,3 +198,55 @@ def test_nan_input_exits_39(gpu, tmp_path):
     assert rc == 39, (out, err)
     assert out.splitlines()[-1] == "Error: Not a number value detected."
     assert (tmp_path / "out-0.n").exists() and not (tmp_path / "out-100.n").exists()
+
+
+C1 = """# graphene crystal, 512 x 512, one field
+I\tin.n
+\tout-#.n
+\t100\t[\t1000\t]\t17\t17
+M\t1
+l\t0.0
+a\t-0.25
+b\t1.0
+g\t1.0
+d\t1.0
+e\t
+s\t0.2
+n\t1.0
+o\t1.0
+p\t0.0
+m\t
+R\t1\t1000\t{dx}\t{dy}\t1.0\t0\t1\t1
+"""
+
+
+def test_c1_graphene_512_end_to_end(gpu, tmp_path):
+    """The single-component graphene configuration through the pfc-relax
+    binary: 512^2, alpha = -0.25, beta = gamma = delta = 1, dx = dy = a/8, a
+    one-mode honeycomb ansatz at mean density -0.2 (conserved dynamics keep
+    it there; the oracle relaxes it to a honeycomb of 4997 rings, while a
+    mean of +0.2 melts) with seeded noise +-0.01, dt = 1, 1000 steps, the
+    free energy every 100 steps.  Every printed fave must match the oracle's
+    relax() to 1e-12 relative, every nave to 1e-14, and the final density
+    to 1e-10 relative L2."""
+    N = 512
+    dx = LATTICE_A / 8
+    f = crystal(N, N, dx, dx, -0.2, 0.2, LATTICE_A, sym=3, theta=0.0, noise=0.01, seed=1)
+    write_density(tmp_path / "in.n", f, dx, dx)
+    (tmp_path / "c1.in").write_text(C1.format(dx=repr(dx), dy=repr(dx)))
+    rc, out, err = run_cli(tmp_path, "c1.in")
+    assert rc == 0, err
+    log = parse_log(out, 1, 17)
+    o = Oracle(graphene_params(c=1.0, dt=1.0), N, N, dx, dx)
+    o.set_fields(f[None])
+    olog, snaps = o.relax(1000, 100, snap_steps=(1000,))
+    assert list(log[:, 0]) == list(range(0, 1001, 100))
+    ef = np.abs(log[:, 3] - olog[:, 3]) / np.abs(olog[:, 3])
+    en = np.abs(log[:, 4] - olog[:, 4])
+    _, n = read_density(tmp_path / "out-1000.n")
+    ed = float(np.linalg.norm(n - snaps[0, 0]) / np.linalg.norm(snaps[0, 0]))
+    print(f"C1 512^2 1000 steps: fave max rel {ef.max():.2e}, nave max abs {en.max():.2e}, "
+          f"density rel L2 {ed:.2e}")
+    assert ef.max() < 1e-12
+    assert en.max() < 1e-14
+    assert ed < 1e-10

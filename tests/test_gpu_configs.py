@@ -1,6 +1,7 @@
-"""The measured configurations at their full sizes on the GPU: the 2048^2
-graphene polycrystal in FP64 and the 8192^2 three-field graphene/hBN
-heterostructure in both forms.  Each runs a few steps against the CPU oracle
+"""The measured configurations at their full sizes on the GPU: the 512^2
+graphene crystal run end to end through the pfc-relax binary, the 2048^2
+graphene polycrystal in FP64, and the 4096^2 and 8192^2 three-field
+graphene/hBN heterostructures in both forms.  Each runs a few steps against the CPU oracle
 (which is too slow for the full runs) and then longer through properties
 that do not depend on the size: conservation of the mean, decrease of the
 free energy, and the mixed form's distance from FP64."""
@@ -47,6 +48,47 @@ def test_polycrystal_2048_fp64(gpu):
             assert abs(ne[0] - ng[0]) <= 1e-13 * abs(ng[0])      # the mean is conserved
         assert all(b < a for a, b in zip(energies, energies[1:])), energies
         assert g.nan_step() == -1
+
+
+def test_heterostructure_4096(gpu):
+    """4096^2, three fields: the benchmark's workload (hetero.in parameters,
+    Gaussian width a/2, seeded 50/50 patchwork of ~256-point domains, noise
+    +-0.01, conserved, dt = 0.25).  FP64 and mixed against the oracle over 2
+    steps (the first a refresh step), evaluate against the oracle, then mixed
+    against FP64 over 100 steps.  This covers the padded column pitch of the
+    full grid and the row pass's XCD block remap at the benchmarked size."""
+    N = 4096
+    p = hetero_params(dt=0.25, c=1.0, sigma=2.0 / 7.2552)
+    f = hetero_patchwork(N, N, 0.7, 0.7, domain=256, seed=2024, noise=0.01)
+    o = Oracle(p, N, N, 0.7, 0.7)
+    o.set_fields(f)
+    assert o.steps(0, 2) == 0
+    ref = o.get_fields()
+    fo, no = o.evaluate()
+    del o
+    with Engine(p, N, N, 0.7, 0.7, FP64) as d, Engine(p, N, N, 0.7, 0.7, MIXED) as m:
+        d.set_fields(f)
+        m.set_fields(f)
+        d.step(0, 2)
+        m.step(0, 2)
+        ed, em = rel_l2(d.get_fields(), ref), rel_l2(m.get_fields(), ref)
+        fd, nd = d.evaluate()
+        fm, nm = m.evaluate()
+        print(f"4096^2 U=3 after 2 steps: density FP64 {ed:.2e} mixed {em:.2e}; "
+              f"energy FP64 {abs(fd - fo) / abs(fo):.2e} mixed {abs(fm - fo) / abs(fo):.2e}")
+        assert ed < 1e-12
+        assert em < 2e-6
+        assert abs(fd - fo) <= 1e-12 * abs(fo)
+        assert np.allclose(nd, no, rtol=0, atol=1e-15)
+        assert abs(fm - fo) <= 1e-6 * abs(fo)
+        d.step(2, 98)
+        m.step(2, 98)
+        fd, nd = d.evaluate()
+        fm, nm = m.evaluate()
+        assert abs(fm - fd) <= 1e-6 * abs(fd), (fm, fd)
+        assert np.allclose(nm, nd, rtol=1e-7, atol=0)
+        assert rel_l2(m.get_fields(), d.get_fields()) < 1e-3
+        assert d.nan_step() == -1 and m.nan_step() == -1
 
 
 def test_heterostructure_8192(gpu):

@@ -299,3 +299,29 @@ def test_relax_cadence():
     log, snaps = o.relax(tend=250, Tprint=100, Topt=0, snap_steps=(0, 100))
     assert list(log[:, 0]) == [0, 100, 200]
     assert snaps.shape == (2, 1, 32, 32)
+
+
+# ------------------------------------------ pair terms and relax cadence
+
+def test_pair_gamma_terms_follow_energy_derivative():
+    """Three uniform fields with alpha_uv, beta_uv, gamma_uv, epsilon_uv all
+    non-zero: each step must equal the scalar exponential-integrator
+    recursion whose explicit part is dF/dn_u (complex-step derivative of the
+    free energy) minus the implicit a_u n_u (src/pfc-relax.c:1191-1242)."""
+    import pins
+    assert pins.pair_gamma_error() < 1e-15
+
+
+def test_relax_event_order_closed_form():
+    """relax() (src/pfc-relax.c:1292-1312) on a linear single-mode model:
+    every printed fave and nave equals the closed-form energy and mean after
+    exactly t steps with the box in force at t (a box search at t comes
+    before the print of t), and every snapshot the closed-form field after
+    exactly t steps (written after the print, before the step)."""
+    import pins
+    r = pins.cadence_errors()
+    assert r["printed"] == [0, 7, 14, 21, 28]
+    assert r["log"] < 1e-13
+    assert r["box"] < 1e-9
+    assert r["snap"] < 2e-12
+    assert r["boxes"][1] != r["boxes"][0] and r["boxes"][2] != r["boxes"][1]
