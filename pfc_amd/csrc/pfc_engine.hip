@@ -203,8 +203,11 @@ int pfc_create(pfc_ctx** out, const pfc_params* p, int Nx, int Ny, int precision
         if (p->W[u] < 0 || p->W[u] > PFC_MAX_MODES) return PFC_EINVAL;
     if (precision != PFC_FP64 && precision != PFC_MIXED) return PFC_EINVAL;
     if (!size_ok(Nx) || !size_ok(Ny)) return PFC_ESIZE;
-    int ndev = 0;
+    int ndev = 0, dev = 0, ncu = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return PFC_EDEVICE;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
+        return PFC_EDEVICE;
 
     pfc_ctx* c = (pfc_ctx*)calloc(1, sizeof(pfc_ctx));
     if (!c) return PFC_ENOMEM;
@@ -213,6 +216,11 @@ int pfc_create(pfc_ctx** out, const pfc_params* p, int Nx, int Ny, int precision
     c->U = p->U;
     c->prec = precision;
     c->p = *p;
+    c->ncu = ncu;
+    {
+        const char* e = getenv("PFC_PERSIST");
+        c->persist = !(e && e[0] == '0');
+    }
     c->nO = 2 * c->U + c->U * (c->U - 1) / 2;
     c->nkn = (size_t)(Nx / 2 + 1) * Ny;   // half spectrum, kx = 0..Nx/2
     c->nr = (size_t)Nx * Ny;

@@ -332,11 +332,8 @@ __device__ __forceinline__ void kupdate(const DevParams* __restrict__ p, T k2, T
 
 // ------------------------------------------------------- real-data helpers
 
-// C2R pre-processing of one row held in natural order (thread t holds
-// X[t + q*TPB], X[0] packed as (DC, Nyquist)): produces Z with
-// IFFT_H(Z)[n] = x[2n] + i x[2n+1], x the unnormalised c2r of X.
-// wt = e^{-2 pi i t/Nx}: e^{-2 pi i k/Nx} for k = t + q*TPB is wt e^{-2 pi i q/(2P)}.
-// row_wt loads it through an opaque zero offset, so that a kernel calling the
+// The pre/post twiddle e^{-2 pi i t/Nx} of lane t (see c2r_pre).  row_wt
+// loads it through an opaque zero offset, so that a kernel calling the
 // helpers several times reloads it rather than keeping it (and the products
 // with the roots) live throughout.
 template <typename T>
@@ -365,12 +362,17 @@ __device__ __forceinline__ cpx<T> wt_value(const cpx<T>* w) {
     return w[z];
 }
 
+// C2R pre-processing of one row held in natural order (thread t holds
+// X[t + q*TPB], X[0] packed as (DC, Nyquist)) in exchange buffer L:
+// produces Z with IFFT_H(Z)[n] = x[2n] + i x[2n+1], x the unnormalised c2r
+// of X.  wt = e^{-2 pi i t/Nx}: e^{-2 pi i k/Nx} for k = t + q*TPB is
+// wt e^{-2 pi i q/(2P)}.
 template <typename T, int H, int TPB, int LS, typename WT>
-__device__ __forceinline__ void c2r_pre(cpx<T> (&v)[H / TPB], cpx<T>* lds, WT wtv, int t) {
+__device__ __forceinline__ void c2r_pre(cpx<T> (&v)[H / TPB], cpx<T>* L, WT wtv, int t) {
     constexpr int P = H / TPB;
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < P; q++) lds[(t + q * TPB) * LS] = v[q];
+    for (int q = 0; q < P; q++) L[(t + q * TPB) * LS] = v[q];
     __syncthreads();
     const cpx<T> wt = wt_value<T>(wtv);
 #pragma unroll
@@ -379,7 +381,7 @@ __device__ __forceinline__ void c2r_pre(cpx<T> (&v)[H / TPB], cpx<T>* lds, WT wt
         // general value is computed for every lane (partner index 0 there)
         // and replaced by a select, so there is no branch
         const cpx<T> a = v[q];
-        const cpx<T> b = cconj(lds[row_partner<H, TPB>(t, q) * LS]);
+        const cpx<T> b = cconj(L[row_partner<H, TPB>(t, q) * LS]);
         const cpx<T> xe = cadd(a, b);
         const cpx<T> w = cmul(wt, root<T>(q, 2 * P, -1));   // e^{-2 pi i k/Nx}
         const cpx<T> xo = cmulc(csub(a, b), w);   // * e^{+2 pi i k/Nx}
@@ -387,22 +389,21 @@ __device__ __forceinline__ void c2r_pre(cpx<T> (&v)[H / TPB], cpx<T>* lds, WT wt
         if (q == 0 && t == 0) v[q] = {a.x + a.y, a.x - a.y};
     }
 }
-
 // R2C post-processing: v holds Z = FFT_H(z), z[n] = x[2n] + i x[2n+1], in
 // natural order; produces X[k] (k < H) with X[0] packed as (DC, Nyquist).
 template <typename T, int H, int TPB, int LS, typename WT>
-__device__ __forceinline__ void r2c_post(cpx<T> (&v)[H / TPB], cpx<T>* lds, WT wtv, int t) {
+__device__ __forceinline__ void r2c_post(cpx<T> (&v)[H / TPB], cpx<T>* L, WT wtv, int t) {
     constexpr int P = H / TPB;
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < P; q++) lds[(t + q * TPB) * LS] = v[q];
+    for (int q = 0; q < P; q++) L[(t + q * TPB) * LS] = v[q];
     __syncthreads();
     const cpx<T> wt = wt_value<T>(wtv);
 #pragma unroll
     for (int q = 0; q < P; q++) {
         // k = 0 only for q = t = 0; selected as in c2r_pre
         const cpx<T> a = v[q];
-        const cpx<T> b = cconj(lds[row_partner<H, TPB>(t, q) * LS]);
+        const cpx<T> b = cconj(L[row_partner<H, TPB>(t, q) * LS]);
         const cpx<T> s = cadd(a, b);   // 2 Xe
         const cpx<T> d = csub(a, b);   // 2i Xo
         const cpx<T> xo2 = {d.y, -d.x};   // 2 Xo
@@ -412,7 +413,6 @@ __device__ __forceinline__ void r2c_post(cpx<T> (&v)[H / TPB], cpx<T>* lds, WT w
         if (q == 0 && t == 0) v[q] = {a.x + a.y, a.x - a.y};   // packed (DC, Nyquist)
     }
 }
-
 // Copies NBYTES (a multiple of 16) from a global buffer at byte offset src
 // into LDS with asynchronous buffer-to-LDS loads, 16 bytes per lane, no
 // registers involved.  Completion is counted like a global load, so issue
@@ -659,6 +659,80 @@ __global__ __launch_bounds__(TPB* TR) PFC_WAVES((row_waves<T, NX>())) void k_row
     const int g = PFC_ROW_XCD ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
     row_step_body<T, NX, TPB, TR, U, REFRESH>(lds, ltab, g, I, J, K, fstride, sI, sJ, twx_, prm,
                                               nan_step, tstep);
+}
+
+// ------------------------------------------------------- persistent passes
+//
+// The FP64 row pass runs as a persistent launch: one grid of resident blocks
+// (the CUs times the blocks one CU holds) walks the row groups instead of one
+// block per group.  A block keeps its twiddle table in LDS for all its groups
+// and its next group's loads start while the CU's waves finish the previous
+// one.  Measured at 4096^2 with three fields: FP64 row pass 391-401 -> 363-365
+// us.  The FP32 row pass (two blocks per CU) went 176 -> 190 us and both
+// column passes lost 14-40 % in the same form, so they stay one block per
+// work item.
+
+// Row group of block b (of nb) at round it, for n row groups, or n when the
+// block is done.  XCD x owns the contiguous range [x n/8, (x+1) n/8) and its
+// blocks j = b/8 take consecutive groups of it in every round, so the CUs of
+// one XCD work on neighbouring rows at any time and their partial lines of
+// the column-major buffers merge in that XCD's L2.
+__device__ __forceinline__ int persist_row_group(int b, int nb, int it, int n) {
+    if (n % 8 != 0 || nb % 8 != 0) {
+        const int g = b + it * nb;
+        return g < n ? g : n;
+    }
+    const int lg = it * (nb / 8) + b / 8;
+    return lg < n / 8 ? (b % 8) * (n / 8) + lg : n;
+}
+
+// The row pass's LDS twiddle table (L = 2^k), gathered and written once.
+template <typename T, int NX, int TPB, int TR>
+__device__ __forceinline__ void row_tab_load(cpx<T>* ltab, const cpx<T>* __restrict__ twx_) {
+    constexpr int NT = TPB * TR;
+    constexpr int NI = row_tab_size<T, NX, TPB, TR>() / NT;
+    if constexpr (BlockFFT<T, NX / 2, TPB, -1, 2, TR>::POW2) {
+        cpx<T> tv[NI];
+        row_tab_gather<T, NX, TPB, TR>(tv, gbuf(twx_), threadIdx.x);
+#pragma unroll
+        for (int i = 0; i < NI; i++) ltab[threadIdx.x + i * NT] = tv[i];
+    }
+}
+
+// Persistent row pass over ngroups row groups of TR rows; the table written
+// before the loop is complete at the first barrier of c2r_pre, which also
+// separates each group's LDS use from the previous group's.
+template <typename T, int NX, int TPB, int TR, int U, bool REFRESH>
+__global__ __launch_bounds__(TPB* TR) PFC_WAVES((row_waves<T, NX>())) void k_row_step_p(
+    const cpx<T>* __restrict__ I, cpx<T>* __restrict__ J, cpx<T>* __restrict__ K,
+    size_t fstride, int sI, int sJ, const cpx<T>* __restrict__ twx_,
+    const DevParams* __restrict__ prm, int* __restrict__ nan_step, int tstep, int ngroups) {
+    __shared__ cpx<T> lds[row_lds_elems<T, NX, TPB, TR>()];
+    __shared__ __attribute__((aligned(16))) cpx<T> ltab[row_tab_size<T, NX, TPB, TR>()];
+    constexpr int H = NX / 2;
+    constexpr int P = H / TPB;
+    constexpr unsigned E = sizeof(cpx<T>);
+    row_tab_load<T, NX, TPB, TR>(ltab, twx_);
+    const int tid = threadIdx.x;
+    const int r = tid % TR;
+    const int t = tid / TR;
+    const GBuf twx = gbuf(twx_);
+    const unsigned istep = lay_row_step(sI, TPB, E);
+    for (int it = 0;; it++) {
+        const int g = persist_row_group(blockIdx.x, gridDim.x, it, ngroups);
+        if (g >= ngroups) break;
+        const int y = g * TR + r;
+        const unsigned ioff = lay_row_off(sI, H, t, y, E);
+        cpx<T> m[U][P];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const GBuf src = gbuf(I + u * fstride);
+#pragma unroll
+            for (int q = 0; q < P; q++) m[u][q] = gload<T>(src, ioff, q * istep);
+        }
+        row_step_core<T, NX, TPB, TR, U, REFRESH>(m, lds + r, ltab, twx, t, y, J, K, fstride, sJ, prm,
+                                                  nan_step, tstep);
+    }
 }
 
 // Column addressing modes of the column passes.  mode 0: an ordinary column

@@ -137,6 +137,8 @@ struct pfc_ctx {
     double *f, *partials, *sums;
     int* nan_flag;
     hipStream_t stream;
+    int ncu;       // compute units of the device
+    int persist;   // FP64 row pass as a persistent launch (PFC_PERSIST=0 turns it off)
     int have_fields, have_box, have_f;
     // pass timing (pfc_timing): 3 events per recorded step
     int timing;

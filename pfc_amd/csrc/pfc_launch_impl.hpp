@@ -8,6 +8,21 @@ namespace pfc {
 
 // Launchers for one element precision T; explicitly instantiated for double
 // and float in their own translation units.
+// Resident blocks of a kernel per CU (1 if the runtime cannot tell).
+inline int occupancy(const void* kernel, int threads) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, threads, 0) != hipSuccess || n < 1) n = 1;
+    return n;
+}
+// Grid of a persistent launch over n work items: every resident block slot,
+// a multiple of 8 (one share per XCD) where that is at least 8, at most n.
+inline int persist_grid(const pfc_ctx* c, int occ, int n) {
+    int nb = c->ncu * occ;
+    if (nb > n) nb = n;
+    if (nb >= 8) nb -= nb % 8;
+    return nb > 0 ? nb : 1;
+}
+
 template <typename T>
 struct LaunchImpl {
     using C = cpx<T>;
@@ -16,9 +31,23 @@ struct LaunchImpl {
     static void row_step(pfc_ctx* c, int t) {
         if constexpr (size_built(NX)) {
             constexpr RowCfg g = row_cfg<T>(NX);
-            hipLaunchKernelGGL((k_row_step<T, NX, g.tpb, g.tr, U, REF>), dim3(c->Ny / g.tr),
-                               dim3(g.tpb * g.tr), 0, c->stream, (const C*)c->I, (C*)c->J, (C*)c->K,
-                               c->nk, c->sI, c->sJ, (const C*)c->twx, c->dprm, c->nan_flag, t);
+            const int ng = c->Ny / g.tr;
+            bool persistent = false;
+            if constexpr (sizeof(T) == 8) {
+                if (c->persist) {
+                    auto k = k_row_step_p<T, NX, g.tpb, g.tr, U, REF>;
+                    static const int occ = occupancy((const void*)k, g.tpb * g.tr);
+                    hipLaunchKernelGGL(k, dim3(persist_grid(c, occ, ng)), dim3(g.tpb * g.tr), 0,
+                                       c->stream, (const C*)c->I, (C*)c->J, (C*)c->K, c->nk, c->sI,
+                                       c->sJ, (const C*)c->twx, c->dprm, c->nan_flag, t, ng);
+                    persistent = true;
+                }
+            }
+            if (!persistent) {
+                hipLaunchKernelGGL((k_row_step<T, NX, g.tpb, g.tr, U, REF>), dim3(ng),
+                                   dim3(g.tpb * g.tr), 0, c->stream, (const C*)c->I, (C*)c->J, (C*)c->K,
+                                   c->nk, c->sI, c->sJ, (const C*)c->twx, c->dprm, c->nan_flag, t);
+            }
         }
     }
     template <int NX, int U>

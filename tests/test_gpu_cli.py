@@ -107,7 +107,8 @@ def test_graphene_run_matches_oracle(gpu, tmp_path, c, topt):
     print(f"c={c} topt={topt}: box {box:.2e} fave {fav:.2e} nave {nav:.2e} density {errs}")
     if topt == 0:
         # no box search: the north star's bars (energy 1e-12, density 1e-10)
-        assert box == 0.0 and fav < 1e-12 and nav < 1e-14
+        # (the box is printed at 14 decimals: equal up to that rounding)
+        assert box < 1e-13 and fav < 1e-12 and nav < 1e-14
         assert max(errs) < 1e-10, errs
     else:
         # Each box search fits a parabola through energies whose second
