@@ -149,6 +149,22 @@ int pfc_smooth(const double* in, double* out, int Nx, int Ny, double dx, double 
 int pfc_fields(const double* density, int W, int H, double dx, double dy, int fold, double sk,
                double sb, double power, double* phi, double* chi);
 
+/* Gaussian smoothing as pfc_smooth, on device buffers and without
+ * allocating anything.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  in and out are device pointers to
+ * Ny x Nx doubles (they may be the same buffer); stream is a hipStream_t
+ * (NULL: the null stream), on which the work is queued asynchronously.  All
+ * work memory is one static device array per GPU of
+ * PFC_NOALLOC_SCRATCH_BYTES (held from the library's load until the process
+ * ends), which limits Nx * Ny to PFC_NOALLOC_MAX_POINTS (2048 x 2048); larger
+ * grids return PFC_ESIZE.  Every call on a GPU shares that GPU's array:
+ * calls that overlap on one GPU (other threads, other streams) race. */
+#define PFC_NOALLOC_MAX_POINTS 4194304
+#define PFC_NOALLOC_SCRATCH_BYTES 34603008
+int pfc_smooth_noalloc(const double* in, double* out, int Nx, int Ny, double dx, double dy,
+                       double sigma, int precision, void* stream);
+
 /* ---- batch form: many independent systems stepped together ------------
  * No counterpart in the reference, which relaxes one system per run
  * (src/pfc-relax.c:1391-1434); a batch is that run for many systems at once.

@@ -25,6 +25,7 @@ SYMBOLS = (
     "pfc_get_fields", "pfc_get_energy", "pfc_nan_step", "pfc_timing", "pfc_timing_read",
     "pfc_destroy", "pfc_strerror", "pfc_smooth", "pfc_fields",
     "pfc_batch_create", "pfc_batch_step", "pfc_batch_shape", "pfc_batch_destroy",
+    "pfc_smooth_noalloc",
 )
 
 BATCH_AUTOTUNE = 1
@@ -63,6 +64,8 @@ def load():
     L.pfc_destroy.argtypes = [vp]
     L.pfc_smooth.argtypes = [dp, dp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                              ctypes.c_double, ctypes.c_int]
+    L.pfc_smooth_noalloc.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                     ctypes.c_double, ctypes.c_double, ctypes.c_int, vp]
     L.pfc_fields.argtypes = [dp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                              ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, dp, dp]
     L.pfc_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int]
@@ -166,7 +169,7 @@ class Engine:
         return s.value
 
 
-__all__ = ["Engine", "Batch", "PfcError", "load", "strerror", "smooth", "fields", "FP64", "MIXED",
+__all__ = ["Engine", "Batch", "PfcError", "load", "strerror", "smooth", "smooth_noalloc", "fields", "FP64", "MIXED",
            "LIB_PATH", "BIN_PATH", "SMOOTH_PATH", "FIELDS_PATH", "SYMBOLS", "BATCH_AUTOTUNE",
            "BATCH_NO_GRAPH"]
 
@@ -183,6 +186,14 @@ def smooth(field, dx, dy, sigma, precision=FP64):
     if rc != 0:
         raise PfcError(rc, "pfc_smooth")
     return out
+
+
+def smooth_noalloc(in_ptr, out_ptr, Nx, Ny, dx, dy, sigma, precision=FP64, stream=None):
+    """pfc_smooth_noalloc on device buffers given by address (in_ptr, out_ptr:
+    Ny x Nx doubles); queued on stream (a hipStream_t handle, None for the
+    null stream).  Returns the status code."""
+    return load().pfc_smooth_noalloc(ctypes.c_void_p(in_ptr), ctypes.c_void_p(out_ptr), Nx, Ny, dx, dy,
+                                     sigma, precision, ctypes.c_void_p(stream or 0))
 
 
 def fields(density, dx, dy, fold, sk, sb, power=None):
