@@ -62,6 +62,11 @@ def survey_bytes(Nx, Ny, U, real_size):
     return U * (16 * Nx * Ny + 96 * (Nx // 2 + 1) * Ny) * real_size / 8
 
 
+def progress(msg):
+    """One line on stderr per finished phase of the benchmark."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
 def relax_window(g, t0, nsteps, tprint):
     """Steps t0 .. t0 + nsteps - 1 as relax() runs them (src/pfc-relax.c:
     1292-1312): an evaluate (the free energy and means that print_output
@@ -107,21 +112,36 @@ def run_gpu(params, fields, Nx, Ny, dx, dy, precision, steps, warmup, tprint=TPR
     return g, wall, ms, n, nev, s0
 
 
-def accuracy_study(params, fields, Nx, Ny, dx, dy, nsteps, checkpoints):
+def accuracy_study(params, fields, Nx, Ny, dx, dy, nsteps, checkpoints, atoms=True):
     """Mixed form against the FP64 form from the same start over the
     workload's full length: relative L2 of the densities and relative error
-    of the free energy per area at each checkpoint.  Beside it, the FP64 form
-    started from the FP32-rounded initial fields (a perturbation of the
-    size of one FP32 rounding, ~3e-8 relative) against the FP64 form: how far
-    the dynamics themselves carry an FP32-level difference."""
+    of the free energy per area at each checkpoint.  Beside it two FP64
+    controls against the FP64 form: (1) started from the FP32-rounded initial
+    fields (one FP32 rounding, ~3e-8 relative), and (2) perturbed after every
+    step by relative noise of the size of one mixed step's error (measured
+    here: one mixed step and one FP64 step from the same FP64 state), the
+    round-off the mixed form injects per step, in FP64 arithmetic.  If (2)
+    strays as far as the mixed form, the distance is the relaxation's
+    sensitivity to any per-step error of that size, not a defect of the FP32
+    arithmetic."""
     from pfc_amd import FP64, MIXED
     from pfc_amd.engine import Engine
     gm = Engine(params, Nx, Ny, dx, dy, MIXED)
     gd = Engine(params, Nx, Ny, dx, dy, FP64)
     gp = Engine(params, Nx, Ny, dx, dy, FP64)
+    gq = Engine(params, Nx, Ny, dx, dy, FP64)
+    # one mixed step against one FP64 step from the same state
+    gm.set_fields(fields)
+    gd.set_fields(fields)
+    gm.step(1, 1)
+    gd.step(1, 1)
+    x1 = gd.get_fields()
+    one_step = float(np.linalg.norm(gm.get_fields() - x1) / np.linalg.norm(x1))
+    del x1
     gm.set_fields(fields)
     gd.set_fields(fields)
     gp.set_fields(fields.astype(np.float32).astype(np.float64))
+    gq.set_fields(fields)
     t = 0
     rows = []
     last = [c for c in checkpoints if c <= nsteps][-1]
@@ -130,41 +150,60 @@ def accuracy_study(params, fields, Nx, Ny, dx, dy, nsteps, checkpoints):
             break
         for g in (gm, gd, gp):
             g.step(t, c - t)
+        gp.nan_step()
+        progress(f"accuracy {Nx}x{Ny}: mixed, FP64 and rounded-start runs at step {c}")
+        for s in range(t, c):
+            gq.step(s, 1)
+            gq.perturb(one_step, 1000003 * s + 17)
+            if (s + 1) % 1000 == 0:
+                gq.nan_step()
+                progress(f"accuracy {Nx}x{Ny}: perturbed FP64 run at step {s + 1}")
         t = c
         fm, _ = gm.evaluate()
         fd, _ = gd.evaluate()
         fp, _ = gp.evaluate()
+        fq, _ = gq.evaluate()
         xd = gd.get_fields()
         nd = np.linalg.norm(xd)
         xm = gm.get_fields()
         em = float(np.linalg.norm(xm - xd) / nd)
         xp = gp.get_fields()
         ep = float(np.linalg.norm(xp - xd) / nd)
+        xq = gq.get_fields()
+        eq = float(np.linalg.norm(xq - xd) / nd)
         row = {"step": c, "energy_per_area_rel": abs(fm - fd) / abs(fd),
                "density_rel_l2": em, "fave_fp64": fd,
                "fp64_from_fp32_rounded_start": {"energy_per_area_rel": abs(fp - fd) / abs(fd),
-                                                "density_rel_l2": ep}}
-        if c == last:
-            row["atoms"] = atom_positions(xd, xm, xp, dx, dy)
-        del xm, xp, xd
+                                                "density_rel_l2": ep},
+               "fp64_perturbed_every_step": {"energy_per_area_rel": abs(fq - fd) / abs(fd),
+                                             "density_rel_l2": eq}}
+        if c == last and atoms:
+            row["atoms"] = atom_positions(xd, xm, xp, dx, dy, xq)
+        progress(f"accuracy {Nx}x{Ny}: checkpoint {c} done")
+        del xm, xp, xd, xq
         rows.append(row)
-    for g in (gm, gd, gp):
+    for g in (gm, gd, gp, gq):
         g.close()
-    return rows
+    return {"one_step_mixed_vs_fp64_density_rel_l2": one_step,
+            "perturbation_per_step_rel": one_step, "checkpoints": rows}
 
 
-def atom_positions(xd, xm, xp, dx, dy):
+def atom_positions(xd, xm, xp, dx, dy, xq):
     """Atoms of the summed density (the coordinator's conversion, lattice
     constant 1) of the FP64 run, and the displacement of each mixed-run and
-    perturbed-run atom from its nearest FP64 atom, in lattice constants."""
+    control-run atom from its nearest FP64 atom, in lattice constants."""
     from pfc_amd import LATTICE_A
     from pfc_amd.coordinator import compare_atoms, coordinate
+    if not all(np.isfinite(x).all() for x in (xd, xm, xp, xq)):
+        return None
     ad, box, nr, fl = coordinate(xd.sum(axis=0), dx, dy, LATTICE_A, 1.0)
-    am, _, _, _ = coordinate(xm.sum(axis=0), dx, dy, LATTICE_A, 1.0)
-    ap, _, _, _ = coordinate(xp.sum(axis=0), dx, dy, LATTICE_A, 1.0)
-    return {"field": "sum of the 3 densities", "unit": "lattice constants", "rings_fp64": nr,
-            "coordinator_flags_fp64": fl, "mixed": compare_atoms(ad, am, box),
-            "fp64_from_fp32_rounded_start": compare_atoms(ad, ap, box)}
+    out = {"field": "sum of the 3 densities", "unit": "lattice constants", "rings_fp64": nr,
+           "coordinator_flags_fp64": fl}
+    for name, x in (("mixed", xm), ("fp64_from_fp32_rounded_start", xp), ("fp64_perturbed_every_step", xq)):
+        a, _, _, _ = coordinate(x.sum(axis=0), dx, dy, LATTICE_A, 1.0)
+        out[name] = compare_atoms(ad, a, box)
+        progress(f"atoms of the {name} run compared")
+    return out
 
 
 def cpu_model():
@@ -340,6 +379,9 @@ def main(argv=None):
                     help="run only the batched-relaxation leg and print its JSON object")
     ap.add_argument("--other-steps", type=int, default=100,
                     help="steps per form of the 2048^2 and 8192^2 configurations (0: skip)")
+    ap.add_argument("--c4-accuracy-steps", type=int, default=0,
+                    help="run only the 8192^2 mixed-vs-FP64 comparison over this many steps "
+                         "(checkpoints 1000, 5000, 20000) and print its JSON object")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported: the whole "
@@ -348,6 +390,15 @@ def main(argv=None):
 
     if a.batch_only:
         print(json.dumps(batch_bench(a.batch_systems, a.batch_steps, 10)))
+        return 0
+    if a.c4_accuracy_steps > 0:
+        from pfc_amd import hetero_params
+        from pfc_amd.inputs import hetero_patchwork
+        p = hetero_params(dt=0.25, c=1.0, sigma=2.0 / 7.2552)
+        f = hetero_patchwork(8192, 8192, 0.7, 0.7, n_domains=128, seed=8192, noise=0.01)
+        r = accuracy_study(p, f, 8192, 8192, 0.7, 0.7, a.c4_accuracy_steps, (1000, 5000, 20000),
+                           atoms=False)
+        print(json.dumps({"workload": "hetero 3-field 8192x8192, 128 domains, dt=0.25", **r}))
         return 0
 
     from pfc_amd import FP64, MIXED, hetero_params
@@ -365,6 +416,7 @@ def main(argv=None):
     res = {}
     for name, prec in (("fp64", FP64), ("mixed", MIXED)):
         g, wall, ms, n, nev, s0 = run_gpu(params, fields, Nx, Ny, dx, dy, prec, a.steps, a.warmup)
+        progress(f"{name} windows done: {1e3 * wall / a.steps:.4f} ms per step")
         esz = 16 if prec == FP64 else 8
         col_ms = ms[1] / max(n, 1)
         row_ms = ms[0] / max(n, 1)
@@ -394,12 +446,16 @@ def main(argv=None):
     batch = None
     if a.batch_systems > 0:
         batch = batch_bench(a.batch_systems, a.batch_steps, 10)
+        progress("batch done")
 
     others = other_configs(a.other_steps) if a.other_steps > 0 else None
+    if others:
+        progress("other configurations done")
 
     cpu = None
     if a.cpu_budget > 0:
         cpu = cpu_baseline(params, fields, Nx, Ny, dx, dy, a.cpu_budget)
+        progress("CPU baseline done")
 
     def kernel_roof(r, pname, esz):
         dom = "column" if r["col_ms"] >= r["row_ms"] else "row"
@@ -449,7 +505,9 @@ def main(argv=None):
                   "step_model_GBps": m["step_model_gbs"],
                   "roofline": kernel_roof(m, "mixed", 8),
                   "accuracy_note": "energy per area against FP64 is within 1e-6 over the first "
-                                   "~100 steps of this random start only; see mixed_vs_fp64_run"},
+                                   "~100 steps of this random start only; mixed_vs_fp64_run sets "
+                                   "it beside an FP64 run perturbed every step by the mixed "
+                                   "form's one-step error"},
         "speedup_mixed_vs_fp64": m["gps"] / d["gps"],
         "mixed_vs_fp64_error": {"density_rel_l2": err_l2, "energy_per_area_rel": err_f,
                                 "after_steps": after},

@@ -119,6 +119,17 @@ int pfc_nan_step(pfc_ctx* ctx, int* step);
 int pfc_timing(pfc_ctx* ctx, int enable);
 int pfc_timing_read(pfc_ctx* ctx, double* ms, int* nsteps);
 
+/* Measurement hook with no counterpart in the reference: multiplies every
+ * real and imaginary part of the step state (the row transforms of the
+ * density, and the spectral state except its columns kx = 0 and kx = Nx/2,
+ * whose anti-Hermitian part the unstable modes amplify ~1e15-fold between
+ * refreshes; the mean density is thus left exact) by 1 + rel * r, r a zero-mean,
+ * unit-variance uniform deviate drawn from a counter hash of (seed, index).
+ * Applied after each step it emulates a per-step round-off of relative size
+ * rel in an otherwise FP64 run, the control that separates the mixed form's
+ * FP32 round-off from the sensitivity of a relaxation to any such error. */
+int pfc_perturb(pfc_ctx* ctx, double rel, unsigned long long seed);
+
 /* Releases everything.  Replaces clean() (:1315-1389). */
 int pfc_destroy(pfc_ctx* ctx);
 

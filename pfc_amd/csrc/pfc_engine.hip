@@ -162,6 +162,30 @@ int dalloc(void** p, size_t bytes) {
     return PFC_OK;
 }
 
+// Multiplies every real and imaginary part x[i] by 1 + rel * r_i, r_i a
+// zero-mean, unit-variance uniform deviate from a counter hash of (seed, i).
+// With nkn > 0, x is the spectral state (U fields of nkn complex entries,
+// column kx at kx * Ny) and its columns kx = 0 and kx = H are left alone:
+// their anti-Hermitian part is invisible to the density and grows with the
+// unstable modes until the next refresh, so noise there would be amplified
+// ~1e15-fold (the FP32 form projects those columns on their Hermitian part
+// for the same reason).
+template <typename T>
+__global__ void k_perturb(T* x, size_t n, size_t nkn, int Ny, double rel, unsigned long long seed) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        if (nkn) {
+            const size_t j = (i / 2) % nkn;
+            if (j < (size_t)Ny || j >= nkn - (size_t)Ny) continue;
+        }
+        unsigned long long z = seed + 0x9E3779B97F4A7C15ull * (i + 1);   // splitmix64
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        const double u = (double)(z >> 11) * (1.0 / 9007199254740992.0);   // [0, 1)
+        x[i] = (T)((double)x[i] * (1.0 + rel * 1.7320508075688772 * (2.0 * u - 1.0)));
+    }
+}
+
 // Materialise the densities in mreal; if K, also refresh Nh from them.
 int materialise(pfc_ctx* c, bool refresh) {
     if (c->prec == PFC_FP64) {
@@ -363,6 +387,25 @@ int pfc_step(pfc_ctx* c, int t0, int nsteps) {
         if (c->prec == PFC_FP64) Launch<double>::col_step_any(c, ref);
         else Launch<float>::col_step_any(c, ref);
         if (ev) HIPCHK(hipEventRecord(ev[2], c->stream));
+    }
+    HIPCHK(hipGetLastError());
+    return PFC_OK;
+}
+
+int pfc_perturb(pfc_ctx* c, double rel, unsigned long long seed) {
+    if (!c || !(rel >= 0.0)) return PFC_EINVAL;
+    if (!c->have_fields) return PFC_ESTATE;
+    const size_t ni = 2 * c->U * c->nk, nn = 2 * c->U * c->nkn;   // real parts of I and Nh
+    if (c->prec == PFC_FP64) {
+        hipLaunchKernelGGL(k_perturb<double>, dim3(2048), dim3(256), 0, c->stream, (double*)c->I, ni,
+                           (size_t)0, c->Ny, rel, seed);
+        hipLaunchKernelGGL(k_perturb<double>, dim3(2048), dim3(256), 0, c->stream, (double*)c->Nh, nn,
+                           c->nkn, c->Ny, rel, seed ^ 0x5bd1e995ull);
+    } else {
+        hipLaunchKernelGGL(k_perturb<float>, dim3(2048), dim3(256), 0, c->stream, (float*)c->I, ni,
+                           (size_t)0, c->Ny, rel, seed);
+        hipLaunchKernelGGL(k_perturb<float>, dim3(2048), dim3(256), 0, c->stream, (float*)c->Nh, nn,
+                           c->nkn, c->Ny, rel, seed ^ 0x5bd1e995ull);
     }
     HIPCHK(hipGetLastError());
     return PFC_OK;

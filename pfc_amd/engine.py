@@ -25,7 +25,7 @@ SYMBOLS = (
     "pfc_get_fields", "pfc_get_energy", "pfc_nan_step", "pfc_timing", "pfc_timing_read",
     "pfc_destroy", "pfc_strerror", "pfc_smooth", "pfc_fields",
     "pfc_batch_create", "pfc_batch_step", "pfc_batch_shape", "pfc_batch_destroy",
-    "pfc_smooth_noalloc",
+    "pfc_smooth_noalloc", "pfc_perturb",
 )
 
 BATCH_AUTOTUNE = 1
@@ -60,6 +60,7 @@ def load():
     L.pfc_get_energy.argtypes = [vp, dp]
     L.pfc_nan_step.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
     L.pfc_timing.argtypes = [vp, ctypes.c_int]
+    L.pfc_perturb.argtypes = [vp, ctypes.c_double, ctypes.c_ulonglong]
     L.pfc_timing_read.argtypes = [vp, dp, ctypes.POINTER(ctypes.c_int)]
     L.pfc_destroy.argtypes = [vp]
     L.pfc_smooth.argtypes = [dp, dp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
@@ -152,6 +153,10 @@ class Engine:
         nave = np.zeros(self.U)
         self._check(load().pfc_evaluate(self._h, ctypes.byref(fave), _dp(nave)), "pfc_evaluate")
         return fave.value, nave
+
+    def perturb(self, rel, seed):
+        """Multiplies the step state by 1 + rel * (unit-variance noise) (pfc_perturb)."""
+        self._check(load().pfc_perturb(self._h, rel, seed), "pfc_perturb")
 
     def timing(self, enable=True):
         self._check(load().pfc_timing(self._h, 1 if enable else 0), "pfc_timing")
