@@ -52,13 +52,18 @@ typedef struct {
     cplx* tw;        /* tw[m] = exp(-2 pi i m / n), computed in long double */
 } fftplan;
 
+/* Stages: radix 4, 2 and 3 first, then every other prime factor of n as a
+ * stage of its own, computed as a direct DFT (any n; a prime n is one O(n^2)
+ * stage, which the small grids of the tests afford). */
 static int plan_init(fftplan* pl, int n) {
     int m = n, k = 0;
     if (n < 1) return -1;
     while (m % 4 == 0) { pl->radix[k++] = 4; m /= 4; }
     while (m % 2 == 0) { pl->radix[k++] = 2; m /= 2; }
     while (m % 3 == 0) { pl->radix[k++] = 3; m /= 3; }
-    if (m != 1) return -1;
+    for (int p = 5; m > 1 && p * p <= m; p += 2)
+        while (m % p == 0) { pl->radix[k++] = p; m /= p; }
+    if (m > 1) pl->radix[k++] = m;
     pl->n = n;
     pl->nr = k;
     pl->tw = (cplx*)malloc(sizeof(cplx) * (size_t)n);
@@ -98,6 +103,32 @@ static void fft_exec(const fftplan* pl, cplx* x, cplx* work, int sign) {
         const int R = pl->radix[s];
         const int nR = n / R;
         const long twstep = n / (Ns * R);
+        if (R > 4) {
+            /* direct DFT of a prime radix: y[s] = sum_r v[r] w^(r s), w = e^{sign 2 pi i/R} */
+            cplx v[R], y[R];
+            const long wstep = n / R;
+            for (int j = 0; j < nR; j++) {
+                const int k = j % Ns;
+                for (int r = 0; r < R; r++) {
+                    v[r] = src[j + r * nR];
+                    if (r > 0 && k > 0) v[r] = cmul(v[r], twid(pl, (long)k * r * twstep, sign));
+                }
+                for (int q = 0; q < R; q++) {
+                    cplx acc = v[0];
+                    for (int r = 1; r < R; r++) {
+                        const cplx t = cmul(v[r], twid(pl, (long)((r * (long)q) % R) * wstep, sign));
+                        acc.re += t.re;
+                        acc.im += t.im;
+                    }
+                    y[q] = acc;
+                }
+                const int base = (j / Ns) * Ns * R + k;
+                for (int r = 0; r < R; r++) dst[base + r * Ns] = y[r];
+            }
+            cplx* t = src; src = dst; dst = t;
+            Ns *= R;
+            continue;
+        }
         for (int j = 0; j < nR; j++) {
             const int k = j % Ns;
             cplx v[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};

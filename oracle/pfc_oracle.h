@@ -25,7 +25,7 @@ typedef struct oracle oracle;
 /* ---- FFT building blocks (FFTW conventions: forward e^{-2 pi i jk/n},
  *      backward e^{+...}, both unnormalised) ---- */
 
-/* 1D complex DFT of length n (n = 2^a 3^b); data as interleaved re,im.
+/* 1D complex DFT of any length n >= 1; data as interleaved re,im.
  * sign = -1 forward, +1 backward. */
 int oracle_fft1d(int n, const double* in, double* out, int sign);
 

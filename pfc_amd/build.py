@@ -34,7 +34,7 @@ HIP_FLAGS = [
 ]
 HIP_UNITS = ["pfc_launch_f64.hip", "pfc_launch_f32.hip", "pfc_engine.hip", "pfc_fields.hip",
              "pfc_batch_f32_row.hip", "pfc_batch_f32_col.hip", "pfc_batch_f64_row.hip",
-             "pfc_batch_f64_col.hip", "pfc_batch.hip", "pfc_noalloc.hip"]
+             "pfc_batch_f64_col.hip", "pfc_batch.hip", "pfc_noalloc.hip", "pfc_generic.hip"]
 HOST_SRCS = ["pfc_relax_main.c", "pfc_io.c"]
 
 

@@ -444,6 +444,7 @@ int pfc_batch_create(pfc_batch** out, pfc_ctx* const* ctxs, int n, int flags) {
         const pfc_ctx* c = ctxs[i];
         if (!c || c->prec != ctxs[0]->prec) return PFC_EINVAL;
         if (c->U != 1 && c->U != 3) return PFC_EINVAL;
+        if (c->generic) return PFC_ESIZE;
         if (!batch_length(c->Nx) || !batch_length(c->Ny)) return PFC_ESIZE;
         if (c->sI != ilog2(c->Ny) || c->sJ != ilog2(c->Ny)) return PFC_EINVAL;
     }

@@ -128,6 +128,7 @@ int upload_params(pfc_ctx* c) {
 }
 
 void free_all(pfc_ctx* c) {
+    gen_destroy(c);
     void* bufs[] = {c->I, c->J, c->K, c->Nh, c->O, c->mreal, c->R, c->twx, c->twy,
                     c->ktab, c->f, c->partials, c->sums, c->nan_flag, c->dprm, c->n0};
     for (void* b : bufs)
@@ -187,7 +188,13 @@ __global__ void k_perturb(T* x, size_t n, size_t nkn, int Ny, double rel, unsign
 }
 
 // Materialise the densities in mreal; if K, also refresh Nh from them.
+// (The generic path keeps them there.)
 int materialise(pfc_ctx* c, bool refresh) {
+    if (c->generic) {
+        if (refresh) gen_refresh(c);
+        HIPCHK(hipGetLastError());
+        return PFC_OK;
+    }
     if (c->prec == PFC_FP64) {
         Launch<double>::row_c2r_any(c, c->I, c->sI, c->mreal, refresh ? c->K : nullptr, c->U);
         if (refresh) Launch<double>::col_fwd_any(c, c->K, c->sJ, c->Nh, c->U, c->hp.inv_NxNy);
@@ -212,8 +219,9 @@ const char* pfc_strerror(int code) {
         case PFC_EDEVICE: return "HIP runtime error";
         case PFC_ENOMEM: return "device allocation failed";
         case PFC_ESIZE:
-            return "grid size not supported by this build (Nx and Ny: 2^k in [64, 8192], or 384, 768, "
-                   "1536)";
+            return "grid size not supported by this build (each of Nx, Ny must be at least 2 and its "
+                   "transform must fit one block's LDS: lengths whose prime factors are at most 13 up "
+                   "to 4864 (FP64) or 9728 (mixed), others up to 2048 (FP64) or 4096 (mixed))";
         case PFC_ESTATE: return "call out of order";
         default: return "unknown error";
     }
@@ -222,11 +230,14 @@ const char* pfc_strerror(int code) {
 int pfc_create(pfc_ctx** out, const pfc_params* p, int Nx, int Ny, int precision) {
     if (!out || !p) return PFC_EINVAL;
     *out = nullptr;
-    if (p->U < 1 || p->U > kMaxU) return PFC_EINVAL;
+    if (p->U < 1 || p->U > PFC_MAX_FIELDS) return PFC_EINVAL;
     for (int u = 0; u < p->U; u++)
         if (p->W[u] < 0 || p->W[u] > PFC_MAX_MODES) return PFC_EINVAL;
     if (precision != PFC_FP64 && precision != PFC_MIXED) return PFC_EINVAL;
-    if (!size_ok(Nx) || !size_ok(Ny)) return PFC_ESIZE;
+    // the tuned kernels where they exist, else the generic path
+    const bool generic = !(size_ok(Nx) && size_ok(Ny) && p->U <= kMaxU);
+    const size_t esz0 = precision == PFC_FP64 ? 16 : 8;
+    if (generic && !(gen_plannable(Nx, esz0) && gen_plannable(Ny, esz0))) return PFC_ESIZE;
     int ndev = 0, dev = 0, ncu = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return PFC_EDEVICE;
     if (hipGetDevice(&dev) != hipSuccess ||
@@ -257,6 +268,8 @@ int pfc_create(pfc_ctx** out, const pfc_params* p, int Nx, int Ny, int precision
     // field stride of I, J, K, O: the largest of their layouts (padded strides)
     c->nk = std::max({lay_elems(c->sC, Nx / 2, Ny, c->esz), lay_elems(c->sI, Nx / 2, Ny, c->esz),
                       lay_elems(c->sJ, Nx / 2, Ny, c->esz)});
+    c->generic = generic;
+    if (generic) c->nk = c->nkn;   // the transposed half-spectrum layout
     const size_t rsz = c->esz / 2;
     c->dx = 1.0;
     c->dy = 1.0;
@@ -293,8 +306,15 @@ int pfc_create(pfc_ctx** out, const pfc_params* p, int Nx, int Ny, int precision
         free(c);
         return PFC_ENOMEM;
     }
+    if (generic && (rc = gen_create(c)) != PFC_OK) {
+        free_all(c);
+        free(c);
+        return rc;
+    }
     bool ok = true;
-    if (precision == PFC_FP64) {
+    if (generic) {
+        // the tuned kernels' tables are not used
+    } else if (precision == PFC_FP64) {
         std::vector<cpx<double>> tx, ty;
         fill_twiddles(tx, Nx);
         fill_twiddles(ty, Ny);
@@ -342,16 +362,24 @@ int pfc_set_fields(pfc_ctx* c, const double* fields) {
         HIPCHK(hipMemcpyAsync(c->mreal, fields, n * sizeof(double), hipMemcpyHostToDevice,
                               c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
-        Launch<double>::row_r2c_any(c, c->mreal, c->I, c->sI, c->U);
-        Launch<double>::col_fwd_any(c, c->I, c->sI, c->Nh, c->U, 1.0 / c->Ny);  // I carries 1/Nx
+        if (c->generic) {
+            gen_refresh(c);
+        } else {
+            Launch<double>::row_r2c_any(c, c->mreal, c->I, c->sI, c->U);
+            Launch<double>::col_fwd_any(c, c->I, c->sI, c->Nh, c->U, 1.0 / c->Ny);  // I carries 1/Nx
+        }
     } else {
         std::vector<float> tmp(n);
         for (size_t i = 0; i < n; i++) tmp[i] = (float)fields[i];
         HIPCHK(hipMemcpyAsync(c->mreal, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice,
                               c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
-        Launch<float>::row_r2c_any(c, c->mreal, c->I, c->sI, c->U);
-        Launch<float>::col_fwd_any(c, c->I, c->sI, c->Nh, c->U, 1.0 / c->Ny);  // I carries 1/Nx
+        if (c->generic) {
+            gen_refresh(c);
+        } else {
+            Launch<float>::row_r2c_any(c, c->mreal, c->I, c->sI, c->U);
+            Launch<float>::col_fwd_any(c, c->I, c->sI, c->Nh, c->U, 1.0 / c->Ny);  // I carries 1/Nx
+        }
         // the k = 0 state in FP64: each field's mean, summed in long double
         std::vector<double> m0(2 * c->U, 0.0);
         for (int u = 0; u < c->U; u++) {
@@ -381,6 +409,13 @@ int pfc_step(pfc_ctx* c, int t0, int nsteps) {
             ev = c->ev + 3 * c->nrec++;
             HIPCHK(hipEventRecord(ev[0], c->stream));
         }
+        if (c->generic) {
+            // one sequence of launches, timed as the row pass
+            gen_step(c, t);
+            if (ev) HIPCHK(hipEventRecord(ev[1], c->stream));
+            if (ev) HIPCHK(hipEventRecord(ev[2], c->stream));
+            continue;
+        }
         if (c->prec == PFC_FP64) Launch<double>::row_step_any(c, t, ref);
         else Launch<float>::row_step_any(c, t, ref);
         if (ev) HIPCHK(hipEventRecord(ev[1], c->stream));
@@ -395,14 +430,17 @@ int pfc_step(pfc_ctx* c, int t0, int nsteps) {
 int pfc_perturb(pfc_ctx* c, double rel, unsigned long long seed) {
     if (!c || !(rel >= 0.0)) return PFC_EINVAL;
     if (!c->have_fields) return PFC_ESTATE;
-    const size_t ni = 2 * c->U * c->nk, nn = 2 * c->U * c->nkn;   // real parts of I and Nh
+    // real parts of the density's state (the generic path keeps the density
+    // itself) and of Nh
+    const size_t ni = c->generic ? c->U * c->nr : 2 * c->U * c->nk, nn = 2 * c->U * c->nkn;
+    void* dens = c->generic ? c->mreal : c->I;
     if (c->prec == PFC_FP64) {
-        hipLaunchKernelGGL(k_perturb<double>, dim3(2048), dim3(256), 0, c->stream, (double*)c->I, ni,
+        hipLaunchKernelGGL(k_perturb<double>, dim3(2048), dim3(256), 0, c->stream, (double*)dens, ni,
                            (size_t)0, c->Ny, rel, seed);
         hipLaunchKernelGGL(k_perturb<double>, dim3(2048), dim3(256), 0, c->stream, (double*)c->Nh, nn,
                            c->nkn, c->Ny, rel, seed ^ 0x5bd1e995ull);
     } else {
-        hipLaunchKernelGGL(k_perturb<float>, dim3(2048), dim3(256), 0, c->stream, (float*)c->I, ni,
+        hipLaunchKernelGGL(k_perturb<float>, dim3(2048), dim3(256), 0, c->stream, (float*)dens, ni,
                            (size_t)0, c->Ny, rel, seed);
         hipLaunchKernelGGL(k_perturb<float>, dim3(2048), dim3(256), 0, c->stream, (float*)c->Nh, nn,
                            c->nkn, c->Ny, rel, seed ^ 0x5bd1e995ull);
@@ -448,7 +486,9 @@ int pfc_timing_read(pfc_ctx* c, double* ms, int* nsteps) {
 int pfc_evaluate(pfc_ctx* c, double* fave, double* nave) {
     if (!c || !fave || !nave) return PFC_EINVAL;
     if (!c->have_fields || !c->have_box) return PFC_ESTATE;
-    if (c->prec == PFC_FP64) {
+    if (c->generic) {
+        gen_evaluate(c);
+    } else if (c->prec == PFC_FP64) {
         Launch<double>::row_c2r_any(c, c->I, c->sI, c->mreal, c->K, c->U);
         Launch<double>::col_eval_any(c);
         Launch<double>::row_c2r_any(c, c->O, c->sC, c->R, nullptr, c->nO);
@@ -525,9 +565,13 @@ int pfc_smooth(const double* in, double* out, int Nx, int Ny, double dx, double 
                 hipSuccess)
                 rc = PFC_EDEVICE;
             if (!rc) {
-                Launch<double>::row_r2c_any(c, c->mreal, c->I, c->sC, 1);
-                Launch<double>::col_smooth_any(c, c->I, c->O, sigma);
-                Launch<double>::row_c2r_any(c, c->O, c->sC, c->R, nullptr, 1);
+                if (c->generic) {
+                    gen_smooth(c, c->mreal, c->R, sigma);
+                } else {
+                    Launch<double>::row_r2c_any(c, c->mreal, c->I, c->sC, 1);
+                    Launch<double>::col_smooth_any(c, c->I, c->O, sigma);
+                    Launch<double>::row_c2r_any(c, c->O, c->sC, c->R, nullptr, 1);
+                }
                 if (hipGetLastError() != hipSuccess ||
                     hipMemcpyAsync(out, c->R, n * sizeof(double), hipMemcpyDeviceToHost,
                                    c->stream) != hipSuccess ||
@@ -541,9 +585,13 @@ int pfc_smooth(const double* in, double* out, int Nx, int Ny, double dx, double 
                                c->stream) != hipSuccess)
                 rc = PFC_EDEVICE;
             if (!rc) {
-                Launch<float>::row_r2c_any(c, c->mreal, c->I, c->sC, 1);
-                Launch<float>::col_smooth_any(c, c->I, c->O, sigma);
-                Launch<float>::row_c2r_any(c, c->O, c->sC, c->R, nullptr, 1);
+                if (c->generic) {
+                    gen_smooth(c, c->mreal, c->R, sigma);
+                } else {
+                    Launch<float>::row_r2c_any(c, c->mreal, c->I, c->sC, 1);
+                    Launch<float>::col_smooth_any(c, c->I, c->O, sigma);
+                    Launch<float>::row_c2r_any(c, c->O, c->sC, c->R, nullptr, 1);
+                }
                 if (hipGetLastError() != hipSuccess ||
                     hipMemcpyAsync(tmp.data(), c->R, n * sizeof(float), hipMemcpyDeviceToHost,
                                    c->stream) != hipSuccess ||

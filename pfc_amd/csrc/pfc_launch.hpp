@@ -139,6 +139,8 @@ struct pfc_ctx {
     hipStream_t stream;
     int ncu;       // compute units of the device
     int persist;   // FP64 row pass as a persistent launch (PFC_PERSIST=0 turns it off)
+    int generic;   // shape or U outside the tuned kernels: the generic path (pfc_generic.hip)
+    void* gen;     // its plans and tables
     int have_fields, have_box, have_f;
     // pass timing (pfc_timing): 3 events per recorded step
     int timing;
@@ -148,6 +150,19 @@ struct pfc_ctx {
 };
 
 namespace pfc {
+
+// The generic path (pfc_generic.hip): any Nx, Ny whose transforms fit one
+// block's LDS and any U <= PFC_MAX_FIELDS.  The density lives in mreal and
+// the half spectra in the transposed (Nx/2 + 1) x Ny layout (nk = nkn).
+struct GenPlan;
+bool gen_plan(int L, size_t esz, GenPlan* p);     // false: L has no generic plan
+bool gen_plannable(int L, size_t esz);
+int gen_create(pfc_ctx* c);                       // plans and device tables
+void gen_destroy(pfc_ctx* c);
+void gen_refresh(pfc_ctx* c);                     // Nh = FFT2(m) / (Nx Ny)
+void gen_step(pfc_ctx* c, int t);                 // one step t
+void gen_evaluate(pfc_ctx* c);                    // up to the energy partial sums
+void gen_smooth(pfc_ctx* c, const void* in, void* out, double sigma);
 
 // Launchers for one element precision T.  Only declared here, so that a unit
 // that launches (the engine) compiles no kernel; pfc_launch_impl.hpp defines

@@ -41,8 +41,11 @@ def test_strerror_and_create_without_device():
     from pfc_amd import graphene_params
     p = graphene_params()
     h = ctypes.c_void_p()
-    # invalid arguments are rejected before any device call
-    assert lib.pfc_create(ctypes.byref(h), ctypes.byref(p), 100, 64, 0) == -4
+    # invalid arguments are rejected before any device call: a prime length
+    # whose Bluestein transform (M = 32768) exceeds one block's LDS, and a
+    # one-point axis
+    assert lib.pfc_create(ctypes.byref(h), ctypes.byref(p), 9973, 64, 0) == -4
+    assert lib.pfc_create(ctypes.byref(h), ctypes.byref(p), 64, 1, 0) == -4
     assert lib.pfc_create(ctypes.byref(h), ctypes.byref(p), 64, 64, 7) == -1
     bad = graphene_params()
     bad.U = 9

@@ -30,7 +30,8 @@ def brute_dft(x, sign):
     return (W * xl[None, :]).sum(axis=1)
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 4, 6, 8, 12, 16, 24, 32, 48, 64, 96, 128, 192])
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 6, 8, 12, 16, 24, 32, 48, 64, 96, 128, 192,
+                               5, 7, 10, 15, 25, 49, 97, 100, 121, 143])
 @pytest.mark.parametrize("sign", [-1, 1])
 def test_fft1d_vs_brute_force(n, sign):
     rng = np.random.default_rng(n)
@@ -49,7 +50,8 @@ def test_fft1d_vs_numpy(n):
     assert np.max(np.abs(OL.fft1d(x, 1) - np.fft.ifft(x) * n)) < 1e-12 * math.sqrt(n)
 
 
-@pytest.mark.parametrize("Ny,Nx", [(8, 8), (12, 16), (16, 6), (64, 32), (48, 96)])
+@pytest.mark.parametrize("Ny,Nx", [(8, 8), (12, 16), (16, 6), (64, 32), (48, 96),
+                                   (35, 45), (9, 7), (20, 13), (97, 10)])
 def test_r2c_c2r_2d(Ny, Nx):
     rng = np.random.default_rng(Nx * Ny)
     a = rng.standard_normal((Ny, Nx))
