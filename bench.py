@@ -299,6 +299,11 @@ def batch_bench(nsys, steps, warmup, seed=96):
     wall1 = time.perf_counter() - t0
     for e in engines:
         e.close()
+    # algorithmic bytes of one step over all systems: the row and column
+    # passes' bytes of each (mixed form: FP32 complex elements, 8 B)
+    step_bytes = sum(row_bytes(n, n, U, 8) + col_bytes(n, n, U, 8) for n, U in specs)
+    gbs = step_bytes / (wall / steps) / 1e9
+    gbs1 = step_bytes / (wall1 / steps) / 1e9
     return {
         "workload": f"{nsys} systems, sides drawn from {list(BATCH_SIDES)} (seed {seed}), "
                     f"{sum(1 for _, u in specs if u == 1)} one-field / "
@@ -310,6 +315,12 @@ def batch_bench(nsys, steps, warmup, seed=96):
         "steps": steps,
         "one_by_one": {"value": gp * steps / wall1, "ms_per_step": 1e3 * wall1 / steps},
         "speedup_vs_one_by_one": wall1 / wall,
+        "roofline": {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": gbs / PEAK_HBM_GBS, "one_by_one_frac": gbs1 / PEAK_HBM_GBS,
+                     "algorithmic_bytes_per_step": step_bytes,
+                     "note": "every system's row- and column-pass bytes per step (each pass "
+                             "reads and writes its fields' spectral buffers once) over the "
+                             "wall-clock step time of the whole batch"},
         "autotune_s": tune_s,
         "shapes": shapes,
         "input_generation_s": gen_s,

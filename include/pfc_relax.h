@@ -189,13 +189,18 @@ int pfc_smooth_noalloc(const double* in, double* out, int Nx, int Ny, double dx,
  * chunk's row and column passes are launches of one block per row group or
  * column block of all its systems, chained on one of four streams, and unless
  * PFC_BATCH_NO_GRAPH, blocks of 20 steps (single steps where t0 or nsteps do
- * not align) are replayed from captured HIP graphs.  PFC_BATCH_AUTOTUNE times both
+ * not align) are replayed from captured HIP graphs.  PFC_BATCH_AUTOTUNE times the
  * FFT shapes of every (length, U) and pass present -- points per thread 8 or
  * 16 (U = 1) and 8 or 4 (U = 3) for 2^k lengths, 12 or 6 (U = 1) and 6 or 12
- * (U = 3) for 3*2^k -- once per process, and times chunk budgets of 8 to 64
- * MiB for the batch; without it the first shape is used unless this process
- * has tuned the class, and 16 MiB chunks.  The tuning passes write only the
- * contexts' scratch buffers.  Destroy the batch before its contexts. */
+ * (U = 3) for 3*2^k, and the first of these with half the rows (columns) per
+ * block; shapes whose kernel spills registers are left out where one without
+ * spills exists -- once per process, and times chunk budgets of 8 to 64 MiB
+ * and two ways of spreading the chunks over the streams for the batch;
+ * without it the first shape is used unless this process has tuned the
+ * class, with 16 MiB chunks spread by work.  The tuning passes write only the
+ * contexts' scratch buffers.  Destroy the batch before its contexts.
+ * (Environment: PFC_BATCH_CHUNK_MB and PFC_BATCH_POLICY (0 or 1) fix the
+ * budget and the spreading; PFC_BATCH_VERBOSE=1 prints the choices.) */
 #define PFC_BATCH_AUTOTUNE 1
 #define PFC_BATCH_NO_GRAPH 2
 typedef struct pfc_batch pfc_batch;

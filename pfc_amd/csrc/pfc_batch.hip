@@ -12,7 +12,14 @@
 // blocks and 16 MiB chunks (the row transforms of a chunk are still in the
 // Infinity Cache when its column pass reads them) bring the batch to 1.55;
 // one block per work item instead of persistent blocks walking the items with
-// a grid stride, 1.50 against 1.67 on the same box.
+// a grid stride, 1.50 against 1.67 on the same box.  Later, on other boxes:
+// 1.36 against 1.42 one by one; descriptors read through the constant address
+// space (no spills left in the FP32 kernels) 1.25 against 1.42; the third,
+// half-block candidate and the stream policy in the tuning 1.27 against 1.47.
+// By class, the batch is 2-4x faster than one by one for sides up to 512 and
+// within 10 % either way for 1536 and 2048 (three-field 2048^2 systems run
+// 0.42 ms batched against 0.39 one by one; running those one after another on
+// one stream (policy 1) made the whole batch slower, 1.35 ms).
 //
 // A batch borrows existing contexts: their buffers, parameters, box tables and
 // NaN flags.  The contexts stay usable through the one-system calls (fields
@@ -21,6 +28,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
 #include <cstdlib>
 #include <map>
 #include <tuple>
@@ -40,7 +48,8 @@ __global__ void k_batch_tick(int* tdev, int nsteps) {
 
 namespace {
 
-// the two candidate classes of (length n, fields U), the default first
+// the k-th candidate class of (length n, fields U), the default first; -1
+// past the last
 int class_of(int n, int U, int k) {
     int found = 0;
     for (int c = 0; c < kBatchClasses; c++)
@@ -86,6 +95,11 @@ struct pfc_batch {
     std::vector<int> gstream;   // stream of each chunk (entry) of a step
     double chunk_bytes = 16.0 * (1 << 20);
     bool chunk_fixed = false;
+    // stream policy: 0 spreads every chunk over the streams by work; 1 runs
+    // the chunks larger than the budget (systems that alone exceed it) one
+    // after another on stream 0 and spreads the rest over the other streams
+    int policy = 0;
+    bool policy_fixed = false;
     int* tdev = nullptr;
     int* tnan = nullptr;      // NaN flag of the tuning passes
     double* tn0 = nullptr;    // k = 0 states of the tuning passes (8 doubles per system)
@@ -291,7 +305,12 @@ int build_step(pfc_batch* b) {
     double load[kBatchStreams] = {};
     b->gstream.clear();
     for (double w : b->rows.work) {
-        const int i = (int)(std::min_element(load, load + kBatchStreams) - load);
+        int i;
+        if (b->policy == 1 && w > b->chunk_bytes) i = 0;
+        else {
+            const int lo = b->policy == 1 ? 1 : 0;
+            i = (int)(std::min_element(load + lo, load + kBatchStreams) - load);
+        }
         b->gstream.push_back(i);
         load[i] += w;
     }
@@ -332,7 +351,37 @@ void choose_classes(pfc_batch* b, bool row_pass, bool tune) {
         const pfc_ctx* c = b->ctx[i];
         groups[{row_pass ? c->Nx : c->Ny, c->U}].push_back(i);
     }
-    for (auto& g : groups) classes[g.first] = class_of(g.first.first, g.first.second, 0);
+    // candidates whose kernel spills are dropped where a candidate without
+    // spills exists; the first remaining one is the default
+    // (candidates with the same tile as an earlier one, as halving a single
+    // column per block gives, are skipped)
+    auto candidates = [&](int n, int U) {
+        std::vector<int> all, clean;
+        for (int k = 0; k < kBatchCandidates; k++) {
+            const int c = class_of(n, U, k);
+            if (c < 0) continue;
+            bool dup = false;
+            for (int a : all)
+                dup = dup || (kBatchClass[a].p == kBatchClass[c].p &&
+                              (row_pass ? batch_tr<T>(a) == batch_tr<T>(c) : batch_tc<T>(a) == batch_tc<T>(c)));
+            if (dup) continue;
+            all.push_back(c);
+            const int sc = row_pass ? BatchLaunch<T>::row_scratch(c) : BatchLaunch<T>::col_scratch(c);
+            if (sc == 0) clean.push_back(c);
+        }
+        return clean.empty() ? all : clean;
+    };
+    for (auto& g : groups) classes[g.first] = candidates(g.first.first, g.first.second)[0];
+    // PFC_BATCH_CANDIDATE=k: the k-th candidate of the class table wherever
+    // it exists (spilling or not, no tuning), for tests of every kernel shape
+    if (const char* e = getenv("PFC_BATCH_CANDIDATE")) {
+        const int k = atoi(e);
+        for (auto& g : groups) {
+            const int c = class_of(g.first.first, g.first.second, k);
+            if (c >= 0) classes[g.first] = c;
+        }
+        return;
+    }
     for (auto& g : groups) {
         const auto key = std::make_tuple(b->prec, row_pass ? 0 : 1, g.first.first, g.first.second);
         auto hit = tuned().find(key);
@@ -341,10 +390,15 @@ void choose_classes(pfc_batch* b, bool row_pass, bool tune) {
             continue;
         }
         if (!tune) continue;
+        const std::vector<int> cand = candidates(g.first.first, g.first.second);
+        if (cand.size() < 2) {
+            tuned()[key] = classes[g.first];
+            continue;
+        }
         float best = 1e30f;
-        for (int k = 0; k < 2; k++) {
+        for (int cls : cand) {
             std::map<std::pair<int, int>, int> trial = classes;
-            trial[g.first] = class_of(g.first.first, g.first.second, k);
+            trial[g.first] = cls;
             const float t = time_pass<T>(b, row_pass, trial, g.second);
             if (t < best) {
                 best = t;
@@ -371,19 +425,38 @@ int build(pfc_batch* b) {
     const bool tune = (b->flags & PFC_BATCH_AUTOTUNE) != 0;
     choose_classes<T>(b, true, tune);
     choose_classes<T>(b, false, tune);
-    // chunk budget: the environment's, else timed (tune), else the default
-    if (tune && !b->chunk_fixed) {
+    // chunk budget and stream policy: the environment's, else timed (tune),
+    // else the defaults
+    if (tune && !(b->chunk_fixed && b->policy_fixed)) {
         float best = 1e30f;
         double pick = b->chunk_bytes;
-        for (double mb : {8.0, 16.0, 32.0, 64.0}) {
-            b->chunk_bytes = mb * (1 << 20);
-            const float t = time_steps<T>(b);
-            if (t < best) {
-                best = t;
-                pick = b->chunk_bytes;
+        int pol = b->policy;
+        std::vector<double> budgets = {8.0, 16.0, 32.0, 64.0};
+        if (b->chunk_fixed) budgets = {b->chunk_bytes / (1 << 20)};
+        std::vector<int> policies = {0, 1};
+        if (b->policy_fixed) policies = {b->policy};
+        for (int po : policies)
+            for (double mb : budgets) {
+                b->chunk_bytes = mb * (1 << 20);
+                b->policy = po;
+                const float t = time_steps<T>(b);
+                if (t < best) {
+                    best = t;
+                    pick = b->chunk_bytes;
+                    pol = po;
+                }
             }
-        }
         b->chunk_bytes = pick;
+        b->policy = pol;
+    }
+    if (getenv("PFC_BATCH_VERBOSE")) {
+        fprintf(stderr, "pfc_batch: %d systems, chunk budget %.0f MiB, stream policy %d\n", b->n,
+                b->chunk_bytes / (1 << 20), b->policy);
+        for (int pass = 0; pass < 2; pass++)
+            for (auto& e : pass == 0 ? b->rclass : b->cclass)
+                fprintf(stderr, "pfc_batch:   %s length %d U=%d: %d points per thread, %d per block\n",
+                        pass == 0 ? "rows" : "columns", e.first.first, e.first.second,
+                        kBatchClass[e.second].p, pass == 0 ? batch_tr<T>(e.second) : batch_tc<T>(e.second));
     }
     const int rc = build_step<T>(b);
     if (rc != PFC_OK) return rc;
@@ -456,6 +529,10 @@ int pfc_batch_create(pfc_batch** out, pfc_ctx* const* ctxs, int n, int flags) {
     if (const char* e = getenv("PFC_BATCH_CHUNK_MB")) {
         b->chunk_bytes = atof(e) * (1 << 20);
         b->chunk_fixed = true;
+    }
+    if (const char* e = getenv("PFC_BATCH_POLICY")) {
+        b->policy = atoi(e) == 1 ? 1 : 0;
+        b->policy_fixed = true;
     }
     if (hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||

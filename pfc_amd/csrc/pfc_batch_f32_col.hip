@@ -4,3 +4,4 @@
 
 template void pfc::BatchLaunch<float>::col(int, bool, const pfc::BatchItem*, int,
                                          const pfc::BatchSys<float>*, hipStream_t);
+template int pfc::BatchLaunch<float>::col_scratch(int);

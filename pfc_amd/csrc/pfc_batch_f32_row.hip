@@ -5,3 +5,4 @@
 template void pfc::BatchLaunch<float>::row(int, bool, const pfc::BatchItem*, int,
                                          const pfc::BatchSys<float>*, const int*, int,
                                          hipStream_t);
+template int pfc::BatchLaunch<float>::row_scratch(int);

@@ -4,3 +4,4 @@
 
 template void pfc::BatchLaunch<double>::col(int, bool, const pfc::BatchItem*, int,
                                          const pfc::BatchSys<double>*, hipStream_t);
+template int pfc::BatchLaunch<double>::col_scratch(int);

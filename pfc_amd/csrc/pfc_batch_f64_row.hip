@@ -5,3 +5,4 @@
 template void pfc::BatchLaunch<double>::row(int, bool, const pfc::BatchItem*, int,
                                          const pfc::BatchSys<double>*, const int*, int,
                                          hipStream_t);
+template int pfc::BatchLaunch<double>::row_scratch(int);

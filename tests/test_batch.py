@@ -70,6 +70,26 @@ def candidates(n, U):
     return (12, 6) if U == 1 else (6, 12)
 
 
+@pytest.mark.parametrize("k", [0, 1, 2])
+@pytest.mark.parametrize("precision", [FP64, MIXED])
+def test_every_kernel_shape_matches_one_system_engines(gpu, monkeypatch, k, precision):
+    """Each candidate tile of the class table (two radices, and half the rows
+    or columns per block), forced for every system, across the t = 100
+    refresh, against pfc_step on each system."""
+    monkeypatch.setenv("PFC_BATCH_CANDIDATE", str(k))
+    es, ref = engines(precision), engines(precision)
+    with Batch([e for e, *_ in es]) as b:
+        b.step(95, 8)
+        if k < 2:
+            for Nx, Ny, U in SYSTEMS:
+                assert b.shape(0, Nx, U) == candidates(Nx, U)[k]
+                assert b.shape(1, Ny, U) == candidates(Ny, U)[k]
+    for (e, *_), (r, *_), s in zip(es, ref, SYSTEMS):
+        r.step(95, 8)
+        err = rel_l2(e.get_fields(), r.get_fields())
+        assert err < (1e-12 if precision == FP64 else 2e-6), (s, k, err)
+
+
 def test_mixed_batch_matches_one_system_engines(gpu):
     es = engines(MIXED)
     ref = engines(MIXED)
