@@ -353,9 +353,13 @@ def other_configs(steps, warmup=3):
         for pname, prec in (("fp64", FP64), ("mixed", MIXED)):
             g, wall, ms, n, nev, _ = run_gpu(params, fields, N, N, h, h, prec, steps, warmup)
             esz = 16 if prec == FP64 else 8
+            rms, cms = ms[0] / max(n, 1), ms[1] / max(n, 1)
             row[pname] = {"ms_per_step": 1e3 * wall / steps,
                           "grid_point_steps_per_s": N * N * steps / wall,
-                          "row_pass_ms": ms[0] / max(n, 1), "column_pass_ms": ms[1] / max(n, 1),
+                          "row_pass_ms": rms, "column_pass_ms": cms,
+                          "row_pass_frac": row_bytes(N, N, U, esz) / (rms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                          "column_pass_frac": col_bytes(N, N, U, esz) / (cms * 1e-3) / 1e9
+                          / PEAK_HBM_GBS,
                           "step_model_frac": survey_bytes(N, N, U, esz // 2) * steps / wall / 1e9
                           / PEAK_HBM_GBS}
             g.close()
