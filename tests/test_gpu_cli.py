@@ -13,7 +13,7 @@ import pytest
 from oracle_lib import Oracle
 from pfc_amd import LATTICE_A, graphene_params, hetero_params
 from pfc_amd.engine import BIN_PATH
-from pfc_amd.inputs import crystal, hetero_patchwork
+from pfc_amd.inputs import crystal, hetero_patchwork, polycrystal
 
 pytestmark = pytest.mark.gpu
 
@@ -251,3 +251,58 @@ def test_c1_graphene_512_end_to_end(gpu, tmp_path):
     assert ef.max() < 1e-12
     assert en.max() < 1e-14
     assert ed < 1e-10
+
+
+C2 = """# 2048 x 2048 graphene polycrystal, box searches every 100 steps
+I\tin.n
+\tout-#.n
+\t100\t[\t100000\t]\t14\t16
+M\t1
+l\t0.0
+a\t-0.25
+b\t1.0
+g\t1.0
+d\t1.0
+e\t
+s\t0.2
+n\t1.0
+o\t1.0
+p\t0.0
+m\t
+R\t1\t200\t{dx}\t{dy}\t1.0\t100\t1\t1
+"""
+
+
+def test_c2_polycrystal_2048_box_search(gpu, tmp_path):
+    """The box-size search (optimize, src/pfc-relax.c:1250-1289) at the
+    polycrystal configuration's size: 2048^2, 64 seeded grains with 2-site
+    boundaries, conserved, dt = 1, the box 1 % too long in y, searches at
+    t = 100 and 200 (the configuration's every 500 steps would take the CPU
+    oracle minutes).  The mean is -0.2, where the honeycomb is stable (at
+    +0.2 it melts and the energy stops depending on the box).  Every printed
+    box and fave must match the oracle's relax() to 1e-12 relative, every
+    nave to 1e-14, and the search must have moved the box.  At this size
+    the search is well conditioned (measured: box 4e-15, fave 2e-13), unlike
+    the 64^2 crystal of test_graphene_run_matches_oracle, whose looser bars
+    rest on tests/test_roundoff_amplification.py."""
+    N = 2048
+    dx = LATTICE_A / 8
+    dy = dx * 1.01
+    f = polycrystal(N, N, dx, dy, 64, -0.2, 0.2, LATTICE_A, sym=3, noise=0.01, boundary=2, seed=64)
+    write_density(tmp_path / "in.n", f, dx, dy)
+    (tmp_path / "c2.in").write_text(C2.format(dx=repr(dx), dy=repr(dy)))
+    rc, out, err = run_cli(tmp_path, "c2.in")
+    assert rc == 0, err
+    log = parse_log(out, 1, 14)
+    o = Oracle(graphene_params(c=1.0, dt=1.0), N, N, dx, dy)
+    o.set_fields(f[None])
+    olog, _ = o.relax(200, 100, Topt=100, xopt=1, yopt=1)
+    o.close()
+    assert list(log[:, 0]) == [0, 100, 200] == list(olog[:, 0])
+    box = float(np.max(np.abs(log[:, 1:3] - olog[:, 1:3]) / olog[:, 1:3]))
+    fav = float(np.max(np.abs(log[:, 3] - olog[:, 3]) / np.abs(olog[:, 3])))
+    nav = float(np.max(np.abs(log[:, 4] - olog[:, 4])))
+    print(f"C2 2048^2 box searches: box {box:.2e} fave {fav:.2e} nave {nav:.2e}; "
+          f"dy {dy!r} -> {log[-1, 2]!r}")
+    assert abs(log[-1, 2] - dy) > 1e-6 * dy       # the search moved the box
+    assert box < 1e-12 and fav < 1e-12 and nav < 1e-14
