@@ -23,8 +23,9 @@ fractions and the whole step's fraction under the three-sweep byte model.
 `cpu_baseline` times the CPU oracle on a bounded sample of the same workload;
 `batch` is the batch form on 96 systems of uneven sizes (2^k and 3*2^k sides
 from 256 to 2048, one or three fields), beside the same systems stepped one
-by one; `other_configs` times the 2048^2 polycrystal and the 8192^2
-heterostructure in both forms.
+by one; `other_configs` times the 2048^2 polycrystal, the 8192^2
+heterostructure and a 1000^2 polycrystal (a side only the generic path
+takes) in both forms.
 """
 from __future__ import annotations
 
@@ -333,8 +334,9 @@ def other_configs(steps, warmup=3):
     """The other measured configurations at full size, both forms: the 2048^2
     graphene polycrystal (64 seeded grains, conserved, dt = 1) and the 8192^2
     three-field heterostructure (dt = 0.25), with the inputs of their GPU
-    tests, timed like the headline (windows starting at a refresh step, one
-    evaluate per 100 steps).  ms per step and grid-point-steps/s."""
+    tests, and a 1000^2 polycrystal on the generic path, timed like the
+    headline (windows starting at a refresh step, one evaluate per 100
+    steps).  ms per step and grid-point-steps/s."""
     from pfc_amd import FP64, MIXED, LATTICE_A, graphene_params, hetero_params
     from pfc_amd.inputs import hetero_patchwork, polycrystal
     out = {}
@@ -348,22 +350,32 @@ def other_configs(steps, warmup=3):
          lambda: (hetero_params(dt=0.25, c=1.0, sigma=2.0 / 7.2552),
                   hetero_patchwork(8192, 8192, 0.7, 0.7, n_domains=128, seed=8192, noise=0.01),
                   8192, 0.7, 3)),
+        # a side the tuned kernels do not cover: the generic path (runtime
+        # mixed-radix transforms, one launch sequence per step, timed whole)
+        ("graphene polycrystal 1000x1000, 1 field, dt=1 (generic path)",
+         lambda: (graphene_params(c=1.0, dt=1.0),
+                  polycrystal(1000, 1000, dx, dx, 16, 0.2, -0.2, LATTICE_A, sym=3, noise=0.01,
+                              boundary=2, seed=1000)[None], 1000, dx, 1)),
     ]
     for name, make in cases:
         params, fields, N, h, U = make()
+        generic = "generic path" in name
         row = {}
         for pname, prec in (("fp64", FP64), ("mixed", MIXED)):
             g, wall, ms, n, nev, _ = run_gpu(params, fields, N, N, h, h, prec, steps, warmup)
             esz = 16 if prec == FP64 else 8
             rms, cms = ms[0] / max(n, 1), ms[1] / max(n, 1)
-            row[pname] = {"ms_per_step": 1e3 * wall / steps,
-                          "grid_point_steps_per_s": N * N * steps / wall,
-                          "row_pass_ms": rms, "column_pass_ms": cms,
-                          "row_pass_frac": row_bytes(N, N, U, esz) / (rms * 1e-3) / 1e9 / PEAK_HBM_GBS,
-                          "column_pass_frac": col_bytes(N, N, U, esz) / (cms * 1e-3) / 1e9
+            r = {"ms_per_step": 1e3 * wall / steps,
+                 "grid_point_steps_per_s": N * N * steps / wall,
+                 "step_model_frac": survey_bytes(N, N, U, esz // 2) * steps / wall / 1e9
+                 / PEAK_HBM_GBS}
+            if not generic:
+                r.update({"row_pass_ms": rms, "column_pass_ms": cms,
+                          "row_pass_frac": row_bytes(N, N, U, esz) / (rms * 1e-3) / 1e9
                           / PEAK_HBM_GBS,
-                          "step_model_frac": survey_bytes(N, N, U, esz // 2) * steps / wall / 1e9
-                          / PEAK_HBM_GBS}
+                          "column_pass_frac": col_bytes(N, N, U, esz) / (cms * 1e-3) / 1e9
+                          / PEAK_HBM_GBS})
+            row[pname] = r
             g.close()
         out[name] = row
         del fields
