@@ -156,7 +156,9 @@ int pfc_smooth(const double* in, double* out, int Nx, int Ny, double dx, double 
  *        Gaussian of width 1/n, each divided by its maximum (fields.c:229-359).
  * fold: order of rotational symmetry (even); sk: width of the complex kernel
  * around |k| = 1; sb: width of the Gaussian smoothing of phi (reciprocal
- * units).  W and H: powers of two in [64, 8192]. */
+ * units).  W and H: any sides whose transforms fit one block's LDS (the
+ * generic transforms: prime factors up to 13 up to 4864, others up to 2048);
+ * 2^k in [64, 8192] and 3 * 2^k run the tuned ones. */
 int pfc_fields(const double* density, int W, int H, double dx, double dy, int fold, double sk,
                double sb, double power, double* phi, double* chi);
 

@@ -25,6 +25,19 @@ struct GenPlan {
     int radix[kGenMaxStages];
 };
 
+// One dimension's plan and device tables (pfc_generic.hip): gen_dim_create
+// plans length L and uploads the twiddles (and Bluestein's chirp tables);
+// PFC_ESIZE if L has no plan within the LDS budget.
+struct GenDim {
+    GenPlan p;
+    void* tw = nullptr;
+    void* chirp = nullptr;
+    void* bhat = nullptr;
+};
+template <typename T>
+int gen_dim_create(GenDim& g, int L);
+void gen_dim_free(GenDim& g);
+
 // LDS elements one generic transform needs (two buffers).
 __host__ __device__ inline int gen_lds_elems(const GenPlan& p) { return 2 * (p.M ? p.M : p.L); }
 
@@ -124,6 +137,11 @@ struct GenTables {
     const cpx<T>* chirp;   // c[n] = e^{-i pi n^2 / L}, n < L
     const cpx<T>* bhat;    // FFT_M(b) / M
 };
+
+template <typename T>
+inline GenTables<T> gen_tables(const GenDim& g) {
+    return {g.p, (const cpx<T>*)g.tw, (const cpx<T>*)g.chirp, (const cpx<T>*)g.bhat};
+}
 
 // L-point transform (unnormalised; DIR = -1 forward, +1 inverse) of the L
 // values in a[0..L); a and b each hold gen_lds_elems/2 elements.  Returns the

@@ -35,6 +35,7 @@
 #include <vector>
 
 #include "../../include/pfc_relax.h"
+#include "fft_generic.hpp"
 #include "pfc_launch.hpp"
 
 namespace pfc {
@@ -206,6 +207,121 @@ __global__ __launch_bounds__(TPB* TC) void k_fcol(const C* __restrict__ src, C* 
     for (int q = 0; q < P; q++) gstore<double>(db, off, q * step, v[q]);
 }
 
+// ------------------------------------------------------------ any size
+//
+// Sides the tuned transforms do not cover run the same passes on the generic
+// runtime-planned transform (fft_generic.hpp): one block of kGenF lanes per
+// row or column with the sequence in LDS (two buffers of gen_lds_elems / 2),
+// the pointwise work in strided loops.  The layouts are those of the tuned
+// passes, with column pitch Hp = H.
+constexpr int kGenF = 256;
+
+template <int MODE>
+__global__ __launch_bounds__(kGenF) void k_gfrow(const C* __restrict__ src, const C* __restrict__ src2,
+                                                 void* __restrict__ dst, int W, int Hp, FieldArgs fa,
+                                                 GenTables<double> gx) {
+    extern __shared__ unsigned char smem_gf[];
+    C* a = (C*)smem_gf;
+    C* b = a + gen_lds_elems(gx.p) / 2;
+    const int y = blockIdx.x;
+    const size_t rix = (size_t)y * W;
+    C* d = (C*)dst;
+    if constexpr (MODE == R_FWD_REAL) {
+        const double m = *fa.pmin;
+        for (int x = threadIdx.x; x < W; x += blockDim.x) a[x] = {fa.p[rix + x] - m, 0.0};
+        const C* r = gen_fft<-1, double>(a, b, gx);
+        for (int k = threadIdx.x; k < W; k += blockDim.x) d[(size_t)k * Hp + y] = r[k];
+        return;
+    }
+    for (int k = threadIdx.x; k < W; k += blockDim.x) a[k] = src[(size_t)k * Hp + y];
+    C* r = gen_fft<+1, double>(a, b, gx);
+    C* o = (r == a) ? b : a;
+    if constexpr (MODE == R_MUL) {
+        // z *= n - min (fields.c:158-164), then forward again
+        const double m = *fa.pmin;
+        for (int x = threadIdx.x; x < W; x += blockDim.x) r[x] = cscale(r[x], fa.p[rix + x] - m);
+        const C* f = gen_fft<-1, double>(r, o, gx);
+        for (int k = threadIdx.x; k < W; k += blockDim.x) d[(size_t)k * Hp + y] = f[k];
+    } else if constexpr (MODE == R_OUT_CPX) {
+        for (int x = threadIdx.x; x < W; x += blockDim.x) d[rix + x] = r[x];
+    } else if constexpr (MODE == R_OUT_RE) {
+        double* dr = (double*)dst;
+        for (int x = threadIdx.x; x < W; x += blockDim.x) dr[rix + x] = r[x].x;
+    } else {
+        // R_GRAD: |grad phi|^power (fields.c:293-301).  |zx| waits in this
+        // row's own slots of dst (the block has read them; each lane reads
+        // back the slots it wrote) while zy is transformed.
+        for (int x = threadIdx.x; x < W; x += blockDim.x) d[(size_t)x * Hp + y] = {hypot(r[x].x, r[x].y), 0.0};
+        for (int k = threadIdx.x; k < W; k += blockDim.x) a[k] = src2[(size_t)k * Hp + y];
+        C* w = gen_fft<+1, double>(a, b, gx);
+        C* o2 = (w == a) ? b : a;
+        for (int x = threadIdx.x; x < W; x += blockDim.x) {
+            const double u = d[(size_t)x * Hp + y].x;
+            const double sv = hypot(w[x].x, w[x].y);
+            w[x] = {pow(u * u + sv * sv, fa.hpow), 0.0};
+        }
+        const C* f = gen_fft<-1, double>(w, o2, gx);
+        for (int k = threadIdx.x; k < W; k += blockDim.x) d[(size_t)k * Hp + y] = f[k];
+    }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kGenF) void k_gfcol(const C* __restrict__ src, C* __restrict__ dst,
+                                                 C* __restrict__ dst2, int W, int NH, int Hp, FieldArgs fa,
+                                                 GenTables<double> gy) {
+    extern __shared__ unsigned char smem_gc[];
+    C* a = (C*)smem_gc;
+    C* b = a + gen_lds_elems(gy.p) / 2;
+    const int kxi = blockIdx.x;
+    const size_t off = (size_t)kxi * Hp;
+    const double kx = wave(kxi, W, fa.dkx);
+    const double kx2 = kx * kx;
+    for (int j = threadIdx.x; j < NH; j += blockDim.x) a[j] = src[off + j];
+    C* r = a;
+    if constexpr (MODE == C_ORIENT || MODE == C_SMOOTH_SAVE || MODE == C_FWD)
+        r = gen_fft<-1, double>(a, b, gy);
+    C* o = (r == a) ? b : a;
+    if constexpr (MODE == C_FWD) {
+        for (int j = threadIdx.x; j < NH; j += blockDim.x) dst[off + j] = r[j];
+        return;
+    }
+    if constexpr (MODE == C_GRAD) {
+        // zx = i kx Z, zy = i ky Z (fields.c:279-291), both back along y; zy
+        // waits in dst2 (each lane reads back the slots it wrote)
+        for (int j = threadIdx.x; j < NH; j += blockDim.x) {
+            const double ky = wave(j, NH, fa.dky);
+            const C v = r[j];
+            dst2[off + j] = {-ky * v.y, ky * v.x};
+            r[j] = {-kx * v.y, kx * v.x};
+        }
+        const C* f = gen_fft<+1, double>(r, o, gy);
+        for (int j = threadIdx.x; j < NH; j += blockDim.x) dst[off + j] = f[j];
+        __syncthreads();   // every lane is done with f before a is rewritten
+        for (int j = threadIdx.x; j < NH; j += blockDim.x) a[j] = dst2[off + j];
+        const C* g = gen_fft<+1, double>(a, b, gy);
+        for (int j = threadIdx.x; j < NH; j += blockDim.x) dst2[off + j] = g[j];
+        return;
+    }
+    for (int j = threadIdx.x; j < NH; j += blockDim.x) {
+        const double ky = wave(j, NH, fa.dky);
+        if constexpr (MODE == C_ORIENT) {
+            // exp(a q^2) e^{i fold phi} / (W H), q = |k| - 1, phi = atan2(ky, kx)
+            // (fields.c:142-154)
+            const double phi = atan2(ky, kx);
+            const double qq = sqrt(kx2 + ky * ky) - 1.0;
+            const double g = exp(fa.a * qq * qq);
+            const double ang = fa.fold * phi;
+            r[j] = cmul(r[j], C{g * cos(ang) * fa.divA, g * sin(ang) * fa.divA});
+        } else {
+            // Gaussian exp(a k^2) / (W H) (fields.c:168-180, 323-337)
+            r[j] = cscale(r[j], exp(fa.a * (kx2 + ky * ky)) * fa.divA);
+        }
+        if constexpr (MODE == C_SMOOTH_SAVE) dst2[off + j] = r[j];
+    }
+    const C* f = gen_fft<+1, double>(r, o, gy);
+    for (int j = threadIdx.x; j < NH; j += blockDim.x) dst[off + j] = f[j];
+}
+
 // Min of a real array, max of a real array or max modulus of a complex array,
 // with fields.c's starting values and strict comparisons (a NaN never wins).
 constexpr int kRedBS = 256;
@@ -279,6 +395,10 @@ struct Job {
     int W, H, Hp;
     hipStream_t s;
     const C *twx, *twy;
+    // generic transforms (sides the tuned kernels do not cover)
+    bool generic;
+    GenTables<double> gx, gy;
+    size_t ldsx, ldsy;   // dynamic LDS bytes of a row / column block
 };
 
 template <int W, int MODE>
@@ -291,6 +411,11 @@ void row_w(const Job& j, const C* src, const C* src2, void* dst, const FieldArgs
 }
 template <int MODE>
 void row(const Job& j, const C* src, const C* src2, void* dst, const FieldArgs& fa) {
+    if (j.generic) {
+        hipLaunchKernelGGL(k_gfrow<MODE>, dim3(j.H), dim3(kGenF), j.ldsx, j.s, src, src2, dst, j.W, j.Hp,
+                           fa, j.gx);
+        return;
+    }
     switch (j.W) {
         case 64: row_w<64, MODE>(j, src, src2, dst, fa); break;
         case 128: row_w<128, MODE>(j, src, src2, dst, fa); break;
@@ -315,6 +440,11 @@ void col_h(const Job& j, const C* src, C* dst, C* dst2, const FieldArgs& fa) {
 }
 template <int MODE>
 void col(const Job& j, const C* src, C* dst, C* dst2, const FieldArgs& fa) {
+    if (j.generic) {
+        hipLaunchKernelGGL(k_gfcol<MODE>, dim3(j.W), dim3(kGenF), j.ldsy, j.s, src, dst, dst2, j.W, j.H,
+                           j.Hp, fa, j.gy);
+        return;
+    }
     switch (j.H) {
         case 64: col_h<64, MODE>(j, src, dst, dst2, fa); break;
         case 128: col_h<128, MODE>(j, src, dst, dst2, fa); break;
@@ -370,16 +500,37 @@ using namespace pfc;
 extern "C" int pfc_fields(const double* density, int W, int H, double dx, double dy, int fold,
                           double sk, double sb, double power, double* phi, double* chi) {
     if (!density || !phi) return PFC_EINVAL;
-    if (!size_ok(W) || !size_ok(H)) return PFC_ESIZE;
+    // the tuned transforms where they exist, else the generic ones
+    const bool generic = !(size_ok(W) && size_ok(H));
+    if (generic && !(gen_plannable(W, E) && gen_plannable(H, E))) return PFC_ESIZE;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return PFC_EDEVICE;
     const size_t n = (size_t)W * H;
-    const int Hp = (int)lay_pitch(ilog2(H), E);
+    const int Hp = generic ? H : (int)lay_pitch(ilog2(H), E);
     const size_t nc = (size_t)W * Hp;
     hipStream_t s;
     if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return PFC_EDEVICE;
     int rc = PFC_OK;
-    {
+    GenDim gdx, gdy;
+    if (generic) {
+        rc = gen_dim_create<double>(gdx, W);
+        if (rc == PFC_OK) rc = gen_dim_create<double>(gdy, H);
+        const size_t lx = (size_t)gen_lds_elems(gdx.p) * E, ly = (size_t)gen_lds_elems(gdy.p) * E;
+#define PFC_GF_LDS(k, b)                                                                         \
+    (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(b))
+        PFC_GF_LDS(k_gfrow<R_FWD_REAL>, lx);
+        PFC_GF_LDS(k_gfrow<R_MUL>, lx);
+        PFC_GF_LDS(k_gfrow<R_OUT_CPX>, lx);
+        PFC_GF_LDS(k_gfrow<R_GRAD>, lx);
+        PFC_GF_LDS(k_gfrow<R_OUT_RE>, lx);
+        PFC_GF_LDS(k_gfcol<C_ORIENT>, ly);
+        PFC_GF_LDS(k_gfcol<C_SMOOTH_SAVE>, ly);
+        PFC_GF_LDS(k_gfcol<C_GRAD>, ly);
+        PFC_GF_LDS(k_gfcol<C_FWD>, ly);
+        PFC_GF_LDS(k_gfcol<C_LEVEL>, ly);
+#undef PFC_GF_LDS
+    }
+    if (rc == PFC_OK) {
         Buffers b;
         double* dp = b.get<double>(n * sizeof(double));
         C* z1 = b.get<C>(nc * E);
@@ -402,7 +553,9 @@ extern "C" int pfc_fields(const double* density, int W, int H, double dx, double
             ok &= hipMemcpyAsync(dp, density, n * sizeof(double), hipMemcpyHostToDevice, s) ==
                   hipSuccess;
             const double pi = 3.14159265358979323846264338327;
-            const Job j{W, H, Hp, s, twx, twy};
+            const Job j{W, H, Hp, s, twx, twy, generic, gen_tables<double>(gdx), gen_tables<double>(gdy),
+                        generic ? (size_t)gen_lds_elems(gdx.p) * E : 0,
+                        generic ? (size_t)gen_lds_elems(gdy.p) * E : 0};
             FieldArgs fa{};
             fa.p = dp;
             fa.pmin = scal;
@@ -451,6 +604,8 @@ extern "C" int pfc_fields(const double* density, int W, int H, double dx, double
             if (!ok) rc = PFC_EDEVICE;
         }
     }
+    gen_dim_free(gdx);
+    gen_dim_free(gdy);
     (void)hipStreamDestroy(s);
     return rc;
 }

@@ -72,13 +72,6 @@ bool gen_plannable(int L, size_t esz) {
 
 namespace {
 
-// One dimension's plan and device tables.
-struct GenDim {
-    GenPlan p;
-    void* tw = nullptr;
-    void* chirp = nullptr;
-    void* bhat = nullptr;
-};
 struct GenCtx {
     GenDim x, y;
 };
@@ -125,6 +118,8 @@ int upload(void** d, const std::vector<cpx<T>>& h) {
                : PFC_EDEVICE;
 }
 
+}  // namespace
+
 template <typename T>
 int gen_dim_create(GenDim& g, int L) {
     if (!gen_plan(L, sizeof(cpx<T>), &g.p)) return PFC_ESIZE;
@@ -165,10 +160,14 @@ void gen_dim_free(GenDim& g) {
         if (p) (void)hipFree(p);
     g.tw = g.chirp = g.bhat = nullptr;
 }
+template int gen_dim_create<double>(GenDim& g, int L);
+template int gen_dim_create<float>(GenDim& g, int L);
+
+namespace {
 
 template <typename T>
 GenTables<T> tables(const GenDim& g) {
-    return {g.p, (const cpx<T>*)g.tw, (const cpx<T>*)g.chirp, (const cpx<T>*)g.bhat};
+    return gen_tables<T>(g);
 }
 
 // ------------------------------------------------------------------ kernels

@@ -56,8 +56,8 @@ int main(int argc, char** argv) {
                    "kernel] [exponent for deformation field]\n\n");
             printf("The order of rotational symmetry is even (2 stripes, 4 square, 6 hexagonal or "
                    "honeycomb); the spreads are in reciprocal units; the density is expected to have "
-                   "its characteristic wave number at k = 1. Grid sides: powers of two in [64, "
-                   "8192].\n");
+                   "its characteristic wave number at k = 1. Grid sides: any whose transform fits one "
+                   "block's LDS (prime factors up to 13: up to 4864; others up to 2048).\n");
         }
         return 0;
     }

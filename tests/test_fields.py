@@ -118,7 +118,10 @@ def rel(a, b):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("W,H", [(64, 64), (128, 64), (64, 256), (512, 512), (1024, 2048), (384, 768),
-                                 (1536, 384)])
+                                 (1536, 384),
+                                 # sides only the generic transforms take: radix 5/3, a
+                                 # prime (Bluestein), 640 x 480, an odd side
+                                 (100, 90), (97, 64), (640, 480), (1000, 125)])
 @pytest.mark.parametrize("power", [1.0, 2.0])
 def test_matches_oracle(gpu, W, H, power):
     rng = np.random.default_rng(W * 7 + H)
