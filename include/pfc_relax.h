@@ -98,12 +98,23 @@ int pfc_step(pfc_ctx* ctx, int t0, int nsteps);
  * Replaces evaluate() incl. its MPI_Allreduce (:987-1130). */
 int pfc_evaluate(pfc_ctx* ctx, double* fave, double* nave);
 
+/* The same fave and nave as pfc_evaluate (to rounding) and the same
+ * re-synchronised state, without the energy density field: the terms with a
+ * filtered field are summed in k space (Parseval), the local ones in real
+ * space.  For the evaluates whose density is never written (relax prints,
+ * :1299-1302, and optimize's samples, :1250-1289, unless a write_energy
+ * follows before the next evaluate).  Afterwards pfc_get_energy fails with
+ * PFC_ESTATE until the next pfc_evaluate.
+ * Replaces evaluate() (:987-1130) where s->f is not read. */
+int pfc_evaluate_sums(pfc_ctx* ctx, double* fave, double* nave);
+
 /* Downloads the current density fields (layout as pfc_set_fields).
  * Feeds write_data (:766-813). */
 int pfc_get_fields(pfc_ctx* ctx, double* fields);
 
 /* Downloads the energy density of the last pfc_evaluate, Ny x Nx values
- * (zeros before the first evaluate).  Feeds write_energy (:816-858). */
+ * (zeros before the first evaluate); PFC_ESTATE if a pfc_evaluate_sums came
+ * after it.  Feeds write_energy (:816-858). */
 int pfc_get_energy(pfc_ctx* ctx, double* f);
 
 /* First step index whose start saw a NaN at field 0, point (0,0), or -1.

@@ -671,7 +671,7 @@ int oracle_optimize(oracle* o, double* d, int xopt, int yopt) {
 /* relax, :1292-1312 (file output replaced by in-memory snapshots) */
 int oracle_relax(oracle* o, int tend, int Tprint, int Topt, int xopt, int yopt,
                  double* log, int max_rows, const int* snap_steps, int n_snap,
-                 double* snaps) {
+                 double* snaps, double* fsnaps) {
     const int U = o->U;
     int rows = 0, ns = 0;
     double d = 0.0001;
@@ -700,6 +700,7 @@ int oracle_relax(oracle* o, int tend, int Tprint, int Topt, int xopt, int yopt,
 #endif
         if (ns < n_snap && t == snap_steps[ns]) {
             oracle_get_fields(o, snaps + (size_t)ns * U * o->nr);
+            if (fsnaps) oracle_get_energy(o, fsnaps + (size_t)ns * o->nr);
             ns++;
         }
 #if ORACLE_MUTANT != 2 && ORACLE_MUTANT != 4

@@ -68,10 +68,12 @@ void oracle_get_box(const oracle* o, double* dx, double* dy);
  * [t, dx, dy, fave, nave_0 .. nave_{U-1}] into log (row stride 4+U), at most
  * max_rows rows.  snap_steps (ascending, n_snap entries) lists steps at which
  * the fields are copied into snaps (n_snap x U x Ny x Nx), mirroring
- * write_data.  Returns the number of rows, or -39 on a NaN. */
+ * write_data, and, if fsnaps is not NULL, the energy density of the last
+ * evaluate into fsnaps (n_snap x Ny x Nx), mirroring write_energy.  Returns
+ * the number of rows, or -39 on a NaN. */
 int oracle_relax(oracle* o, int tend, int Tprint, int Topt, int xopt, int yopt,
                  double* log, int max_rows, const int* snap_steps, int n_snap,
-                 double* snaps);
+                 double* snaps, double* fsnaps);
 
 #ifdef __cplusplus
 }

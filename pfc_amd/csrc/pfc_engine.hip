@@ -288,8 +288,10 @@ int pfc_create(pfc_ctx** out, const pfc_params* p, int Nx, int Ny, int precision
     rc |= dalloc(&c->mreal, c->U * c->nr * rsz);
     rc |= dalloc(&c->R, c->nO * c->nr * rsz);
     rc |= dalloc((void**)&c->f, c->nr * sizeof(double));
-    rc |= dalloc((void**)&c->partials, (size_t)kEnergyBlocks * 2 * (1 + c->U) * sizeof(double));
-    rc |= dalloc((void**)&c->sums, (1 + c->U) * sizeof(double));
+    // energy partials: per block 1 + U sums (f or its local part, each n_u)
+    // and the spectral part of pfc_evaluate_sums
+    rc |= dalloc((void**)&c->partials, (size_t)kEnergyBlocks * 2 * (2 + c->U) * sizeof(double));
+    rc |= dalloc((void**)&c->sums, (2 + c->U) * sizeof(double));
     rc |= dalloc((void**)&c->nan_flag, sizeof(int));
     rc |= dalloc((void**)&c->dprm, sizeof(DevParams));
     rc |= dalloc((void**)&c->n0, 2 * c->U * sizeof(double));
@@ -508,7 +510,32 @@ int pfc_evaluate(pfc_ctx* c, double* fave, double* nave) {
     HIPCHK(hipStreamSynchronize(c->stream));
     *fave = h[0];
     for (int u = 0; u < c->U; u++) nave[u] = h[1 + u];
-    c->have_f = 1;
+    c->f_stale = 0;
+    return PFC_OK;
+}
+
+int pfc_evaluate_sums(pfc_ctx* c, double* fave, double* nave) {
+    if (!c || !fave || !nave) return PFC_EINVAL;
+    if (!c->have_fields || !c->have_box) return PFC_ESTATE;
+    // the densities and the refreshed state, as the full evaluate leaves them
+    const int rc = materialise(c, true);
+    if (rc) return rc;
+    if (c->generic) gen_energy_sums(c);
+    else if (c->prec == PFC_FP64) Launch<double>::energy_sums_any(c);
+    else Launch<float>::energy_sums_any(c);
+    hipLaunchKernelGGL((k_reduce_final<1024>), dim3(1), dim3(1024), 0, c->stream, c->partials,
+                       kEnergyBlocks, 1 + c->U, c->sums, c->hp.inv_NxNy);
+    hipLaunchKernelGGL((k_reduce_final<1024>), dim3(1), dim3(1024), 0, c->stream,
+                       c->partials + (size_t)kEnergyBlocks * 2 * (1 + c->U), kEnergyBlocks, 1,
+                       c->sums + 1 + c->U, 1.0);
+    HIPCHK(hipGetLastError());
+    double h[2 + PFC_MAX_FIELDS];
+    HIPCHK(hipMemcpyAsync(h, c->sums, (2 + c->U) * sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    *fave = h[0] + h[1 + c->U];
+    for (int u = 0; u < c->U; u++) nave[u] = h[1 + u];
+    c->f_stale = 1;
     return PFC_OK;
 }
 
@@ -534,6 +561,7 @@ int pfc_get_fields(pfc_ctx* c, double* fields) {
 
 int pfc_get_energy(pfc_ctx* c, double* f) {
     if (!c || !f) return PFC_EINVAL;
+    if (c->f_stale) return PFC_ESTATE;
     HIPCHK(hipMemcpyAsync(f, c->f, c->nr * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return PFC_OK;

@@ -422,6 +422,24 @@ struct Gen {
             default: energy_u<8>(c); break;
         }
     }
+    template <int U>
+    static void sums_u(pfc_ctx* c) {
+        launch_energy_sums<T, U>((const T*)c->mreal, (const C*)c->Nh, c->nr, c->nkn, c->Nx, c->Ny,
+                                 c->partials, kEnergyBlocks, c->dprm, c->stream);
+    }
+    // the local and spectral energy sums from mreal and Nh
+    static void energy_sums(pfc_ctx* c) {
+        switch (c->U) {
+            case 1: sums_u<1>(c); break;
+            case 2: sums_u<2>(c); break;
+            case 3: sums_u<3>(c); break;
+            case 4: sums_u<4>(c); break;
+            case 5: sums_u<5>(c); break;
+            case 6: sums_u<6>(c); break;
+            case 7: sums_u<7>(c); break;
+            default: sums_u<8>(c); break;
+        }
+    }
     // out = smoothed in (both real, in mreal's layout) via J
     static void smooth(pfc_ctx* c, const void* in, void* out, double sigma) {
         row_r2c(c, in, c->J, 1);
@@ -487,6 +505,10 @@ void gen_step(pfc_ctx* c, int t) {
 void gen_evaluate(pfc_ctx* c) {
     if (c->prec == PFC_FP64) Gen<double>::evaluate(c);
     else Gen<float>::evaluate(c);
+}
+void gen_energy_sums(pfc_ctx* c) {
+    if (c->prec == PFC_FP64) Gen<double>::energy_sums(c);
+    else Gen<float>::energy_sums(c);
 }
 void gen_smooth(pfc_ctx* c, const void* in, void* out, double sigma) {
     if (c->prec == PFC_FP64) Gen<double>::smooth(c, in, out, sigma);

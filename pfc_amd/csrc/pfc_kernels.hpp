@@ -1387,6 +1387,113 @@ __global__ __launch_bounds__(BS) void k_energy(const T* __restrict__ n, const T*
     }
 }
 
+// The free energy's sums without its density field, in two parts that add up
+// to k_energy's sum of f (src/pfc-relax.c:1093-1117).
+//
+// Local part, from the densities alone: lambda n + alpha n^2/2 + gamma n^3/3 +
+// delta n^4/4 per field and alpha_uv n_u n_v + gamma_uv (n_u^2 n_v + n_u n_v^2)/2
+// per pair; per-block compensated partials of that sum and of every n_u, in
+// k_energy's layout.
+template <typename T, int U, int BS>
+__global__ __launch_bounds__(BS) void k_energy_local(const T* __restrict__ n, size_t nr,
+                                                     double* __restrict__ partials,
+                                                     const DevParams* __restrict__ p) {
+    KSum acc[1 + U];
+#pragma unroll
+    for (int i = 0; i <= U; i++) acc[i] = {0.0, 0.0};
+    const double _3 = 1.0 / 3.0;
+    for (size_t ij = (size_t)blockIdx.x * BS + threadIdx.x; ij < nr; ij += (size_t)gridDim.x * BS) {
+        double nv[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) nv[u] = (double)n[u * nr + ij];
+        double fij = 0.0;
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const double nu = nv[u];
+            const double nunu = nu * nu;
+            fij += p->lambda[u] * nu + 0.5 * p->alpha[u][u] * nunu + _3 * p->gamma[u][u] * nu * nunu +
+                   0.25 * p->delta[u] * nunu * nunu;
+#pragma unroll
+            for (int v = u + 1; v < U; v++) {
+                const double nvv = nv[v];
+                fij += p->alpha[u][v] * nu * nvv + 0.5 * p->gamma[u][v] * (nunu * nvv + nu * nvv * nvv);
+            }
+            kadd(acc[1 + u], nu);
+        }
+        kadd(acc[0], fij);
+    }
+    block_ksum<1 + U, BS>(acc);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i <= U; i++) {
+            partials[(size_t)blockIdx.x * 2 * (1 + U) + 2 * i] = acc[i].s;
+            partials[(size_t)blockIdx.x * 2 * (1 + U) + 2 * i + 1] = acc[i].c;
+        }
+    }
+}
+
+// Spectral part: the terms with a filtered field (P_u n_u, the smoothed
+// densities e_u = G n_u, the pair gradient term Q_uv) are bilinear, so by
+// Parseval their grid mean is a sum over k of the state N = FFT2(n)/(Nx Ny):
+//   mean(n_u C2R(m N_v)) = sum over the half spectrum of w m Re(N_u conj N_v)
+// with w = 1 for kx = 0 and kx = Nx/2 (Nx even) and 2 for the columns between:
+// the real-space path's C2R along x takes the interior columns as 2 Re(.) and
+// the real part of the other two, so this holds for any Ny, also where the
+// wavevector convention (j < Ny/2 positive) makes m uneven in ky (odd Ny).
+// Per k: beta_uu/2 P_u |N_u|^2 and (beta_uv M_uv + epsilon_uv G^2)
+// Re(N_u conj N_v) with the real-space path's multipliers (eval_mult).  Nh
+// holds nkn = (Nx/2 + 1) Ny entries per field, kx major.  Per-block
+// compensated partials of the one sum.
+template <typename T, int U, int BS>
+__global__ __launch_bounds__(BS) void k_energy_spec(const cpx<T>* __restrict__ Nh, size_t nkn,
+                                                    int Nx, int Ny, double* __restrict__ partials,
+                                                    const DevParams* __restrict__ p) {
+    KSum acc[1] = {{0.0, 0.0}};
+    for (size_t idx = (size_t)blockIdx.x * BS + threadIdx.x; idx < nkn; idx += (size_t)gridDim.x * BS) {
+        const int kx = (int)(idx / (unsigned)Ny);
+        const int j = (int)(idx - (size_t)kx * Ny);
+        const double kxd = kx * p->dkx;
+        const double ky = ky_of(j, Ny, p->dky);
+        const double k2 = kxd * kxd + ky * ky;
+        double re[U], im[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const cpx<T> z = Nh[u * nkn + idx];
+            re[u] = (double)z.x;
+            im[u] = (double)z.y;
+        }
+        const double G = eval_mult(p, 1, 0, 0.0, k2);
+        const double G2 = G * G;
+        double e = 0.0;
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            e += 0.5 * p->beta[u][u] * eval_mult(p, 0, u, 0.0, k2) * (re[u] * re[u] + im[u] * im[u]);
+#pragma unroll
+            for (int v = u + 1; v < U; v++) {
+                const double m = p->beta[u][v] * eval_mult(p, 2, u, p->qc2[u][v], k2) + p->epsilon[u][v] * G2;
+                e += m * (re[u] * re[v] + im[u] * im[v]);
+            }
+        }
+        const bool once = (kx == 0) || (2 * kx == Nx);
+        kadd(acc[0], once ? e : 2.0 * e);
+    }
+    block_ksum<1, BS>(acc);
+    if (threadIdx.x == 0) {
+        partials[2 * (size_t)blockIdx.x] = acc[0].s;
+        partials[2 * (size_t)blockIdx.x + 1] = acc[0].c;
+    }
+}
+
+// Launches both parts: local partials at partials[0 .. 2 (1 + U) nblk), the
+// spectral ones after them.
+template <typename T, int U>
+void launch_energy_sums(const T* n, const cpx<T>* Nh, size_t nr, size_t nkn, int Nx, int Ny,
+                        double* partials, int nblk, const DevParams* p, hipStream_t s) {
+    hipLaunchKernelGGL((k_energy_local<T, U, 256>), dim3(nblk), dim3(256), 0, s, n, nr, partials, p);
+    hipLaunchKernelGGL((k_energy_spec<T, U, 256>), dim3(nblk), dim3(256), 0, s, Nh, nkn, Nx, Ny,
+                       partials + (size_t)nblk * 2 * (1 + U), p);
+}
+
 // Second stage: one block folds nblk partial pairs per quantity.
 template <int BS>
 __global__ __launch_bounds__(BS) void k_reduce_final(const double* __restrict__ partials,

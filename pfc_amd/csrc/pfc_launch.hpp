@@ -141,7 +141,8 @@ struct pfc_ctx {
     int persist;   // FP64 row pass as a persistent launch (PFC_PERSIST=0 turns it off)
     int generic;   // shape or U outside the tuned kernels: the generic path (pfc_generic.hip)
     void* gen;     // its plans and tables
-    int have_fields, have_box, have_f;
+    int have_fields, have_box;
+    int f_stale;   // the last evaluate was pfc_evaluate_sums: f is older than it
     // pass timing (pfc_timing): 3 events per recorded step
     int timing;
     int nrec;
@@ -162,6 +163,7 @@ void gen_destroy(pfc_ctx* c);
 void gen_refresh(pfc_ctx* c);                     // Nh = FFT2(m) / (Nx Ny)
 void gen_step(pfc_ctx* c, int t);                 // one step t
 void gen_evaluate(pfc_ctx* c);                    // up to the energy partial sums
+void gen_energy_sums(pfc_ctx* c);                 // k_energy_local + k_energy_spec
 void gen_smooth(pfc_ctx* c, const void* in, void* out, double sigma);
 
 // Launchers for one element precision T.  Only declared here, so that a unit
@@ -177,6 +179,7 @@ struct Launch {
     static void col_smooth_any(pfc_ctx* c, const void* src, void* dst, double sigma);
     static void col_eval_any(pfc_ctx* c);
     static void energy_any(pfc_ctx* c);
+    static void energy_sums_any(pfc_ctx* c);   // after the refresh: Nh and mreal current
 };
 
 }  // namespace pfc

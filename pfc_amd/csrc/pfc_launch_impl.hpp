@@ -258,6 +258,19 @@ struct LaunchImpl {
             case 4: energy<4>(c); break;
         }
     }
+    template <int U>
+    static void energy_sums(pfc_ctx* c) {
+        launch_energy_sums<T, U>((const T*)c->mreal, (const C*)c->Nh, c->nr, c->nkn, c->Nx, c->Ny,
+                                 c->partials, kEnergyBlocks, c->dprm, c->stream);
+    }
+    static void energy_sums_any(pfc_ctx* c) {
+        switch (c->U) {
+            case 1: energy_sums<1>(c); break;
+            case 2: energy_sums<2>(c); break;
+            case 3: energy_sums<3>(c); break;
+            case 4: energy_sums<4>(c); break;
+        }
+    }
 };
 
 template <typename T>
@@ -284,5 +297,7 @@ template <typename T>
 void Launch<T>::col_eval_any(pfc_ctx* c) { LaunchImpl<T>::col_eval_any(c); }
 template <typename T>
 void Launch<T>::energy_any(pfc_ctx* c) { LaunchImpl<T>::energy_any(c); }
+template <typename T>
+void Launch<T>::energy_sums_any(pfc_ctx* c) { LaunchImpl<T>::energy_sums_any(c); }
 
 }  // namespace pfc

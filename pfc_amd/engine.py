@@ -21,7 +21,7 @@ FIELDS_PATH = os.path.join(PKG, "bin", "pfc-fields")
 
 # exported symbols of include/pfc_relax.h
 SYMBOLS = (
-    "pfc_create", "pfc_set_fields", "pfc_set_box", "pfc_step", "pfc_evaluate",
+    "pfc_create", "pfc_set_fields", "pfc_set_box", "pfc_step", "pfc_evaluate", "pfc_evaluate_sums",
     "pfc_get_fields", "pfc_get_energy", "pfc_nan_step", "pfc_timing", "pfc_timing_read",
     "pfc_destroy", "pfc_strerror", "pfc_smooth", "pfc_fields",
     "pfc_batch_create", "pfc_batch_step", "pfc_batch_shape", "pfc_batch_destroy",
@@ -56,6 +56,7 @@ def load():
     L.pfc_set_box.argtypes = [vp, ctypes.c_double, ctypes.c_double]
     L.pfc_step.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     L.pfc_evaluate.argtypes = [vp, dp, dp]
+    L.pfc_evaluate_sums.argtypes = [vp, dp, dp]
     L.pfc_get_fields.argtypes = [vp, dp]
     L.pfc_get_energy.argtypes = [vp, dp]
     L.pfc_nan_step.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
@@ -148,10 +149,13 @@ class Engine:
     def step(self, t0, n=1):
         self._check(load().pfc_step(self._h, t0, n), "pfc_step")
 
-    def evaluate(self):
+    def evaluate(self, density=True):
+        """(fave, nave).  density=False takes pfc_evaluate_sums: the same sums
+        without the energy density field that get_energy downloads."""
         fave = ctypes.c_double()
         nave = np.zeros(self.U)
-        self._check(load().pfc_evaluate(self._h, ctypes.byref(fave), _dp(nave)), "pfc_evaluate")
+        fn = "pfc_evaluate" if density else "pfc_evaluate_sums"
+        self._check(getattr(load(), fn)(self._h, ctypes.byref(fave), _dp(nave)), fn)
         return fave.value, nave
 
     def perturb(self, rel, seed):

@@ -11,6 +11,7 @@
  *   PFC_PRECISION=mixed      FP32 transforms/update, FP64 reductions
  *   PFC_ENERGY_COMPAT=1      reproduce evaluate's nu2-for-mu2 alias
  */
+#include <limits.h>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -58,28 +59,60 @@ typedef struct {
     double* host;   /* U x Ny x Nx staging for writes */
 } run_state;
 
-static double energy_at(run_state* s, double dx, double dy) {
+/* evaluate (src/pfc-relax.c:987-1130).  A write_energy at step tw writes the
+ * density of the last evaluate at or before tw, so only an evaluate that is
+ * the last one before such a write needs the density field (pfc_evaluate);
+ * every other one takes the sums alone (pfc_evaluate_sums). */
+static double evaluate(run_state* s, int density) {
     double fave;
-    int rc = pfc_set_box(s->ctx, dx, dy);
-    if (rc) gpu_fail(rc, "pfc_set_box");
-    rc = pfc_evaluate(s->ctx, &fave, s->nave);
-    if (rc) gpu_fail(rc, "pfc_evaluate");
+    const int rc = density ? pfc_evaluate(s->ctx, &fave, s->nave)
+                           : pfc_evaluate_sums(s->ctx, &fave, s->nave);
+    if (rc) gpu_fail(rc, density ? "pfc_evaluate" : "pfc_evaluate_sums");
     return fave;
+}
+
+/* first step after t that evaluates (a print or an optimisation) */
+static int next_evaluate_step(const run_cfg* c, int t) {
+    int te = INT_MAX;
+    if (c->Tprint > 0) {
+        const int k = (t / c->Tprint + 1) * c->Tprint;
+        if (k < te) te = k;
+    }
+    if (c->Topt > 0) {
+        const int k = (t / c->Topt + 1) * c->Topt;
+        if (k < te) te = k;
+    }
+    return te;
+}
+
+/* whether the last evaluate at step t feeds an energy file: the next write
+ * step tw (index nw) has t <= tw < the next evaluating step */
+static int density_due(const run_cfg* c, int t, int nw) {
+    return c->has_fname && nw < c->ntwrite && c->twrite[nw] >= t &&
+           c->twrite[nw] < next_evaluate_step(c, t);
+}
+
+static double energy_at(run_state* s, double dx, double dy, int density) {
+    const int rc = pfc_set_box(s->ctx, dx, dy);
+    if (rc) gpu_fail(rc, "pfc_set_box");
+    return evaluate(s, density);
 }
 
 /* optimize (src/pfc-relax.c:1250-1289): parabola vertex per axis from
  * energies at dx -+ d and dy -+ d, change limited to a quarter of the
- * lattice constant l0 = 4 pi/sqrt(3), then the sampling step adapts. */
-static void optimize(run_state* s, double* d) {
+ * lattice constant l0 = 4 pi/sqrt(3), then the sampling step adapts.
+ * last_density: the last sample's evaluate feeds an energy file. */
+static void optimize(run_state* s, double* d, int last_density) {
     run_cfg* c = s->cfg;
     const double dd = *d;
     const double dxs[5] = {c->dx, c->dx - dd, c->dx + dd, c->dx, c->dx};
     const double dys[5] = {c->dy, c->dy, c->dy, c->dy - dd, c->dy + dd};
     double fs[5] = {0, 0, 0, 0, 0};
+    const int last = c->yopt != 0 ? 4 : (c->xopt != 0 ? 2 : 0);
     for (int i = 0; i < 5; i++) {
         if (c->xopt == 0 && (i == 1 || i == 2)) continue;
         if (c->yopt == 0 && (i == 3 || i == 4)) continue;
-        fs[i] = energy_at(s, dxs[i], dys[i]);
+        fs[i] = energy_at(s, dxs[i], dys[i], i == last && last_density);
     }
     double dx = dxs[0], dy = dys[0];
     if (c->xopt != 0)
@@ -136,11 +169,10 @@ static void relax(run_state* s) {
     int nw = 0;
     int t = 0;
     while (t <= c->tend) {
-        if (c->Topt > 0 && t > 0 && t % c->Topt == 0) optimize(s, &d);
-        if (c->Tprint > 0 && t % c->Tprint == 0) {
-            double fave;
-            const int rc = pfc_evaluate(s->ctx, &fave, s->nave);
-            if (rc) gpu_fail(rc, "pfc_evaluate");
+        const int print = c->Tprint > 0 && t % c->Tprint == 0;
+        if (c->Topt > 0 && t > 0 && t % c->Topt == 0) optimize(s, &d, !print && density_due(c, t, nw));
+        if (print) {
+            const double fave = evaluate(s, density_due(c, t, nw));
             pfc_print_output(c, t, (int)(time(NULL) - s->t0), c->dx, c->dy, fave, s->nave);
         }
         if (nw < c->ntwrite && t == c->twrite[nw]) {

@@ -52,7 +52,7 @@ def lib(path=None):
                                            ctypes.c_int, L.ENERGY_FN, vp]
         L.oracle_relax.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                    ctypes.c_int, dp, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
-                                   ctypes.c_int, dp]
+                                   ctypes.c_int, dp, dp]
         _libs[path] = L
     return _libs[path]
 
@@ -162,15 +162,21 @@ class Oracle:
         self.L.oracle_optimize(self._h, ctypes.byref(dd), xopt, yopt)
         return dd.value
 
-    def relax(self, tend, Tprint, Topt=0, xopt=1, yopt=1, snap_steps=()):
+    def relax(self, tend, Tprint, Topt=0, xopt=1, yopt=1, snap_steps=(), energy=False):
+        """(log, field snapshots), and with energy=True also the energy
+        densities that write_energy would write at the snapshot steps."""
         rows = tend // max(Tprint, 1) + 2
         log = np.zeros((rows, 4 + self.U))
         snaps = np.zeros((max(len(snap_steps), 1), self.U, self.Ny, self.Nx))
+        fsnaps = np.zeros((max(len(snap_steps), 1), self.Ny, self.Nx)) if energy else None
         ss = (ctypes.c_int * max(len(snap_steps), 1))(*snap_steps)
         n = self.L.oracle_relax(self._h, tend, Tprint, Topt, xopt, yopt, _dp(log), rows, ss,
-                               len(snap_steps), _dp(snaps))
+                               len(snap_steps), _dp(snaps),
+                               _dp(fsnaps) if energy else None)
         if n < 0:
             raise FloatingPointError(f"oracle relax failed with code {-n}")
+        if energy:
+            return log[:n], snaps[:len(snap_steps)], fsnaps[:len(snap_steps)]
         return log[:n], snaps[:len(snap_steps)]
 
 

@@ -93,23 +93,30 @@ def test_graphene_run_matches_oracle(gpu, tmp_path, c, topt):
     assert (tmp_path / "run.out").read_text() == out
     o = Oracle(graphene_params(c=float(c)), Nx, Ny, dx, dy)
     o.set_fields(f[None])
-    olog, snaps = o.relax(250, 100, Topt=topt, xopt=1, yopt=1, snap_steps=(0, 100, 250))
+    olog, snaps, fsnaps = o.relax(250, 100, Topt=topt, xopt=1, yopt=1, snap_steps=(0, 100, 250),
+                                  energy=True)
     assert np.array_equal(log[:, 0], olog[:, 0])
     box = float(np.max(np.abs(log[:, 1:3] - olog[:, 1:3]) / olog[:, 1:3]))
     fav = float(np.max(np.abs(log[:, 3] - olog[:, 3]) / np.abs(olog[:, 3])))
     nav = float(np.max(np.abs(log[:, 4] - olog[:, 4])))
-    errs = []
+    errs, ferrs = [], []
     for k, t in enumerate((0, 100, 250)):
         hdr, n = read_density(tmp_path / f"out-{t}.n")
         assert hdr[:2] == [str(Nx), str(Ny)]
         errs.append(float(np.linalg.norm(n - snaps[k, 0]) / np.linalg.norm(snaps[k, 0])))
-        assert (tmp_path / f"out-{t}.f").exists()
-    print(f"c={c} topt={topt}: box {box:.2e} fave {fav:.2e} nave {nav:.2e} density {errs}")
+        # the energy density of the last evaluate at or before t: t's own
+        # print at 0 and 100; at 250 (no print) the print at 200 without a
+        # box search, else the last sample of the search at 250
+        hdr, fe = read_density(tmp_path / f"out-{t}.f")
+        ferrs.append(float(np.linalg.norm(fe - fsnaps[k]) / np.linalg.norm(fsnaps[k])))
+    print(f"c={c} topt={topt}: box {box:.2e} fave {fav:.2e} nave {nav:.2e} density {errs} "
+          f"energy density {ferrs}")
     if topt == 0:
         # no box search: the north star's bars (energy 1e-12, density 1e-10)
         # (the box is printed at 14 decimals: equal up to that rounding)
         assert box < 1e-13 and fav < 1e-12 and nav < 1e-14
         assert max(errs) < 1e-10, errs
+        assert max(ferrs) < 1e-10, ferrs
     else:
         # Each box search fits a parabola through energies whose second
         # difference is ~1e-9 of the energy, so a round-off difference in the
@@ -120,6 +127,7 @@ def test_graphene_run_matches_oracle(gpu, tmp_path, c, topt):
         # differently again.  The bars are set above that spread.
         assert box < 5e-11 and fav < 5e-11 and nav < 1e-14
         assert max(errs) < 1e-9, errs
+        assert max(ferrs) < 1e-9, ferrs
 
 
 HETERO = """I\ta.n\tb.n\tc.n
