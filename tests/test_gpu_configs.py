@@ -10,7 +10,7 @@ import math
 import numpy as np
 import pytest
 
-from oracle_lib import Oracle
+from oracle_lib import Oracle, optimize_with
 from pfc_amd import FP64, MIXED, LATTICE_A, graphene_params, hetero_params
 from pfc_amd.engine import Engine
 from pfc_amd.inputs import hetero_patchwork, polycrystal
@@ -91,12 +91,10 @@ def test_heterostructure_4096(gpu):
         assert d.nan_step() == -1 and m.nan_step() == -1
 
 
-def test_heterostructure_8192(gpu):
-    """8192^2, three fields, 128 seeded domains, dt = 0.25: FP64 and mixed
-    against the oracle for 2 steps, then mixed against FP64 over 100 steps
-    (energy per area within 1e-6, the north star's bound).  The hetero.in
-    parameters diverge at dt = 0.5 (the CPU oracle reaches NaN within 20
-    steps at 256^2), so the demo's dt = 0.25 is used."""
+@pytest.fixture(scope="module")
+def c4():
+    """The 8192^2 heterostructure start (three fields, 128 seeded domains,
+    dt = 0.25) and the oracle after its first 2 steps."""
     N = 8192
     p = hetero_params(dt=0.25, c=1.0, sigma=2.0 / 7.2552)
     f = hetero_patchwork(N, N, 0.7, 0.7, n_domains=128, seed=8192, noise=0.01)
@@ -105,7 +103,17 @@ def test_heterostructure_8192(gpu):
     assert o.steps(0, 2) == 0
     ref = o.get_fields()
     fo, _ = o.evaluate()
-    del o
+    yield dict(N=N, p=p, f=f, o=o, ref=ref, fo=fo)
+    o.close()
+
+
+def test_heterostructure_8192(gpu, c4):
+    """8192^2, three fields, 128 seeded domains, dt = 0.25: FP64 and mixed
+    against the oracle for 2 steps, then mixed against FP64 over 100 steps
+    (energy per area within 1e-6, the north star's bound).  The hetero.in
+    parameters diverge at dt = 0.5 (the CPU oracle reaches NaN within 20
+    steps at 256^2), so the demo's dt = 0.25 is used."""
+    N, p, f, ref, fo = c4["N"], c4["p"], c4["f"], c4["ref"], c4["fo"]
     with Engine(p, N, N, 0.7, 0.7, FP64) as d, Engine(p, N, N, 0.7, 0.7, MIXED) as m:
         d.set_fields(f)
         m.set_fields(f)
@@ -123,3 +131,41 @@ def test_heterostructure_8192(gpu):
         assert np.allclose(nm, nd, rtol=1e-7, atol=0)
         assert rel_l2(m.get_fields(), d.get_fields()) < 1e-3
         assert d.nan_step() == -1 and m.nan_step() == -1
+
+
+def test_c4_box_search_8192(gpu, c4):
+    """The box search of optimize (src/pfc-relax.c:1250-1289) at 8192^2 after
+    the first 2 steps, along x with the reference's first sampling step
+    d = 1e-4: the energies at dx - d, dx and dx + d from the GPU (sums-only
+    evaluates after pfc_set_box) against the oracle's to 1e-12, and the box
+    the search picks from each set of energies.  The parabola's vertex
+    divides the energies' differences by their second difference, so the
+    boxes may differ by up to d * |energy difference| / |second difference|
+    (a few times that), which the test states and checks."""
+    N, p, f, o = c4["N"], c4["p"], c4["f"], c4["o"]
+    d = 1e-4
+    boxes = [(0.7 - d, 0.7), (0.7 + d, 0.7)]
+    fo = {(0.7, 0.7): c4["fo"]}
+    for b in boxes:
+        o.set_box(*b)
+        fo[b] = o.evaluate()[0]
+    o.set_box(0.7, 0.7)
+    with Engine(p, N, N, 0.7, 0.7, FP64) as g:
+        g.set_fields(f)
+        g.step(0, 2)
+        fg = {}
+        for b in [(0.7, 0.7)] + boxes:
+            g.set_box(*b)
+            fg[b] = g.evaluate(density=False)[0]
+        g.set_box(0.7, 0.7)
+    errs = {b: abs(fg[b] - fo[b]) / abs(fo[b]) for b in fo}
+    assert max(errs.values()) <= 1e-12, errs
+    lo, mid, hi = (0.7 - d, 0.7), (0.7, 0.7), (0.7 + d, 0.7)
+    xo, _, _ = optimize_with(N, N, 0.7, 0.7, d, 1, 0, lambda a, b: fo[(a, b)])
+    xg, _, _ = optimize_with(N, N, 0.7, 0.7, d, 1, 0, lambda a, b: fg[(a, b)])
+    second = abs(fo[lo] - 2 * fo[mid] + fo[hi])
+    bound = 4 * d * max(abs(fg[b] - fo[b]) for b in fo) / second + 1e-16
+    print(f"8192^2 box search: energies {max(errs.values()):.2e}, dx oracle {xo!r} GPU {xg!r}, "
+          f"difference {abs(xg - xo):.2e} (bound {bound:.2e}), moved {xo - 0.7:.3e}")
+    assert xo != 0.7
+    assert abs(xg - xo) <= bound, (xg, xo, bound)
