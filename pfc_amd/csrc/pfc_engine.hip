@@ -129,7 +129,7 @@ int upload_params(pfc_ctx* c) {
 
 void free_all(pfc_ctx* c) {
     gen_destroy(c);
-    void* bufs[] = {c->I, c->J, c->K, c->Nh, c->O, c->mreal, c->R, c->twx, c->twy,
+    void* bufs[] = {c->I, c->J, c->K, c->Nh, c->Nh2, c->O, c->mreal, c->R, c->twx, c->twy,
                     c->ktab, c->f, c->partials, c->sums, c->nan_flag, c->dprm, c->n0};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
@@ -255,6 +255,9 @@ int pfc_create(pfc_ctx** out, const pfc_params* p, int Nx, int Ny, int precision
     {
         const char* e = getenv("PFC_PERSIST");
         c->persist = !(e && e[0] == '0');
+        e = getenv("PFC_COLSPLIT");
+        c->colsplit = !(e && e[0] == '0') && precision == PFC_FP64 && !generic &&
+                      Launch<double>::col_split_built(Ny);
     }
     c->nO = 2 * c->U + c->U * (c->U - 1) / 2;
     c->nkn = (size_t)(Nx / 2 + 1) * Ny;   // half spectrum, kx = 0..Nx/2
@@ -284,6 +287,7 @@ int pfc_create(pfc_ctx** out, const pfc_params* p, int Nx, int Ny, int precision
     rc |= dalloc(&c->J, spec);
     rc |= dalloc(&c->K, spec);
     rc |= dalloc(&c->Nh, c->U * c->nkn * c->esz);
+    if (c->colsplit) rc |= dalloc(&c->Nh2, c->U * c->nkn * c->esz);
     rc |= dalloc(&c->O, c->nO * c->nk * c->esz);
     rc |= dalloc(&c->mreal, c->U * c->nr * rsz);
     rc |= dalloc(&c->R, c->nO * c->nr * rsz);
@@ -402,6 +406,7 @@ int pfc_set_fields(pfc_ctx* c, const double* fields) {
 int pfc_step(pfc_ctx* c, int t0, int nsteps) {
     if (!c || nsteps < 0) return PFC_EINVAL;
     if (!c->have_fields || !c->have_box) return PFC_ESTATE;
+    bool alt = false;   // the state is in Nh2 (between the two steps of a pair)
     for (int i = 0; i < nsteps; i++) {
         const int t = t0 + i;
         const bool ref = (t % 100 == 0);
@@ -421,8 +426,18 @@ int pfc_step(pfc_ctx* c, int t0, int nsteps) {
         if (c->prec == PFC_FP64) Launch<double>::row_step_any(c, t, ref);
         else Launch<float>::row_step_any(c, t, ref);
         if (ev) HIPCHK(hipEventRecord(ev[1], c->stream));
-        if (c->prec == PFC_FP64) Launch<double>::col_step_any(c, ref);
-        else Launch<float>::col_step_any(c, ref);
+        // Non-refresh steps go through the per-field column pass in pairs,
+        // Nh -> Nh2 -> Nh, so that the state is in Nh between calls and on
+        // refresh steps; a step without a partner takes the in-place pass.
+        const bool split = c->Nh2 && (alt || (!ref && i + 1 < nsteps && (t + 1) % 100 != 0));
+        if (split) {
+            Launch<double>::col_step_split_any(c, alt ? c->Nh2 : c->Nh, alt ? c->Nh : c->Nh2);
+            alt = !alt;
+        } else if (c->prec == PFC_FP64) {
+            Launch<double>::col_step_any(c, ref);
+        } else {
+            Launch<float>::col_step_any(c, ref);
+        }
         if (ev) HIPCHK(hipEventRecord(ev[2], c->stream));
     }
     HIPCHK(hipGetLastError());

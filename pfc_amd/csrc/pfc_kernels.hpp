@@ -141,6 +141,13 @@ __device__ __forceinline__ int xcd_remap(int b, int n) {
     return (b % 8) * (n / 8) + b / 8;
 }
 
+// The same for any n: XCD x takes the n/8 (+1 for x < n % 8) consecutive
+// indices from x (n/8) + min(x, n % 8).
+__device__ __forceinline__ int xcd_remap_any(int b, int n) {
+    const int x = b % 8, q = n / 8, r = n % 8;
+    return x * q + (x < r ? x : r) + b / 8;
+}
+
 __device__ __forceinline__ double ky_of(int j, int Ny, double dky) {
     return (j < Ny / 2) ? j * dky : (j - Ny) * dky;
 }
@@ -1042,6 +1049,157 @@ __global__ __launch_bounds__(TPB* TC) PFC_WAVES(col_waves<T>()) void k_col_step(
         col_step_body<T, NY, TPB, TC, U, REFRESH, LTAB, LGY>(lds, ltw, nullptr, blockIdx.x, J, K, Nh,
                                                              I, fstride, nstride, H, sI, sJ, twy_,
                                                              ktab_, prm, n0);
+    }
+}
+
+// ------------------------------------------------ per-field column pass
+//
+// The k-space update of field f alone needs the transform of f's own
+// nonlinear term and the old states of all fields at the same k (the
+// explicit couplings of src/pfc-relax.c:1214-1236 read N_v, not M_v).  So
+// the column pass splits into one block per (column, field): each block
+// transforms one field, reads every field's old state from Nsrc and writes
+// its field's new state to Ndst (a second state buffer: the other fields'
+// blocks still read the old one).  A block then holds one field's column in
+// registers instead of U, which lets two FP64 blocks share a CU and overlap
+// one block's memory traffic with the other's transforms.  The U blocks of a
+// column run on one XCD at about the same time (xcd_remap_any over the
+// items), so the repeated state reads are served by its L2.
+
+// kupdate for field f alone: the same operations in the same order as
+// kupdate's for M[f], N[f] (couplings by ascending v != f, then A, B).
+template <int U, typename T>
+__device__ __forceinline__ cpx<T> kupdate_one(const DevParams* __restrict__ p, int f, T k2, T G2,
+                                             cpx<T> M, const cpx<T> (&N)[U]) {
+    const T NxNy = (T)p->NxNy;
+    cpx<T> Nf = N[0];
+#pragma unroll
+    for (int v = 0; v < U; v++) {
+        if (v == f) {
+            Nf = N[v];
+            continue;
+        }
+        const int a = v < f ? v : f, b = v < f ? f : v;
+        T d2 = (T)p->mu2[a][b] - k2;
+        d2 *= d2;
+        const T bd = (T)p->beta[a][b] * d2;
+        const T eg = (T)p->epsilon[a][b] * G2;
+        M.x += (bd * N[v].x + eg * N[v].x) * NxNy;
+        M.y += (bd * N[v].y + eg * N[v].y) * NxNy;
+    }
+    T A, B;
+    op_AB(p, f, k2, A, B);
+    return {A * Nf.x + B * M.x, A * Nf.y + B * M.y};
+}
+
+// Compact twiddle table of the per-field pass: exactly kTab entries (no
+// padding to the block's lanes, so that two FP64 blocks of 4096 fit one
+// CU's LDS beside their exchange buffers).
+template <typename T, int NY, int TPB>
+constexpr int colf_tw_entries() {
+    return BlockFFT<T, NY, TPB, -1, 1, 1>::kTab;
+}
+// state loads of the per-field pass in batches of QB k points (U loads each)
+#ifndef PFC_COLF_QB
+#define PFC_COLF_QB 2
+#endif
+#ifndef PFC_COLF_WAVES
+#define PFC_COLF_WAVES 4
+#endif
+template <typename T, int NY>
+constexpr int colf_waves() { return (sizeof(T) == 8 && NY == 8192) ? 2 : PFC_COLF_WAVES; }
+
+// Per-field column pass of a non-refresh step (L = 2^k columns).  Grid:
+// (H/TC + 1) * U blocks; item = xcd_remap(block), column block item / U,
+// field item % U.  Columns kx = 0 and kx = H as in col_job.
+template <typename T, int NY, int TPB, int TC, int U>
+__global__ __launch_bounds__(TPB* TC) PFC_WAVES((colf_waves<T, NY>())) void k_col_step_f(
+    const cpx<T>* __restrict__ J, const cpx<T>* __restrict__ Nsrc, cpx<T>* __restrict__ Ndst,
+    cpx<T>* __restrict__ I, size_t fstride, size_t nstride, int H, int sI, int sJ,
+    const cpx<T>* __restrict__ twy_, const T* __restrict__ ktab_,
+    const DevParams* __restrict__ prm) {
+    constexpr int P = NY / TPB;
+    constexpr unsigned E = sizeof(cpx<T>);
+    using FFTF = BlockFFT<T, NY, TPB, -1, 1, 1, step_padc<T>()>;
+    using FFTI = BlockFFT<T, NY, TPB, +1, 1, 1, step_padc<T>()>;
+    static_assert(FFTF::POW2, "the per-field pass takes 2^k columns");
+    constexpr int NTAB = colf_tw_entries<T, NY, TPB>();
+    constexpr int NT = TPB * TC;
+    constexpr int NI = (NTAB + NT - 1) / NT;
+    __shared__ cpx<T> lds[col_lds_len<T, NY, TPB>() * TC];
+    __shared__ __attribute__((aligned(16))) cpx<T> ltw[NTAB];
+    const int item = xcd_remap_any(blockIdx.x, gridDim.x);
+    const int cb = item / U, f = item % U;
+    const int tid = threadIdx.x;
+    const int c = tid / TPB;
+    const int t = tid % TPB;
+    cpx<T>* L = lds + c * FFTF::lds_len;
+    const GBuf twy = gbuf(twy_);
+    const GBuf ktab = gbuf(ktab_);
+    const ColJob jb = col_job(cb, c, TC, H);
+    const int kx = jb.kx;
+    const unsigned vr = lay_col_off(sJ, H, jb.mode ? 0 : kx, t, E);
+    const unsigned rstep = lay_col_step(sJ, H, TPB, E);
+    const unsigned vn = ((unsigned)kx * NY + (unsigned)t) * E;
+    const unsigned vi = lay_col_off(sI, H, jb.mode ? 0 : kx, t, E);
+    const unsigned istep = lay_col_step(sI, H, TPB, E);
+
+    // the table gather first, then the field's J (the LDS write of the
+    // table waits for the gather only)
+    cpx<T> tv[NI];
+#pragma unroll
+    for (int i = 0; i < NI; i++) {
+        const int e = tid + i * NT;
+        tv[i] = gload<T>(twy, (unsigned)(e < NTAB ? FFTF::tab_src(e) : 0) * E, 0u);
+    }
+    const T gx = gload_real<T>(ktab, (unsigned)kx * sizeof(T));
+    __builtin_amdgcn_sched_barrier(0);
+    cpx<T> M[P];
+    {
+        const GBuf src = gbuf(J + f * fstride);
+#pragma unroll
+        for (int q = 0; q < P; q++) M[q] = col_in<T>(src, vr, q * rstep, jb.mode);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int i = 0; i < NI; i++) {
+        const int e = tid + i * NT;
+        if (e < NTAB) ltw[e] = tv[i];
+    }
+    __syncthreads();
+    FFTF::run(M, L, StageTab<T>{ltw}, t);
+    if (jb.live) {
+        const double kxd = kx * prm->dkx;
+        const double kx2 = kxd * kxd;
+        constexpr int QB = (P % PFC_COLF_QB == 0) ? PFC_COLF_QB : 1;
+        const GBuf nd = gbuf(Ndst + f * nstride);
+#pragma unroll
+        for (int q0 = 0; q0 < P; q0 += QB) {
+            cpx<T> Nb[QB][U];
+            T gyb[QB];
+#pragma unroll
+            for (int i = 0; i < QB; i++) {
+#pragma unroll
+                for (int v = 0; v < U; v++)
+                    Nb[i][v] = gload<T>(gbuf(Nsrc + v * nstride), vn, (q0 + i) * TPB * E);
+                gyb[i] = gload_real<T>(ktab, (unsigned)(ktab_gy(H) + t + (q0 + i) * TPB) * sizeof(T));
+            }
+#pragma unroll
+            for (int i = 0; i < QB; i++) {
+                const int q = q0 + i;
+                const int j = t + q * TPB;
+                const double ky = ky_of(j, NY, prm->dky);
+                const T k2 = (T)(kx2 + ky * ky);
+                M[q] = kupdate_one<U, T>(prm, f, k2, gx * gyb[i], M[q], Nb[i]);
+                gstore<T, PFC_COL_AUX>(nd, vn, q * TPB * E, M[q]);
+            }
+        }
+    }
+    FFTI::run(M, L, StageTab<T>{ltw}, t);
+    if (jb.live) {
+        const GBuf dst = gbuf(I + f * fstride);
+#pragma unroll
+        for (int q = 0; q < P; q++) col_out<T, PFC_COL_AUX>(dst, vi, q * istep, jb.mode, M[q]);
     }
 }
 

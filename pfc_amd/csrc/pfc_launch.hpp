@@ -139,6 +139,11 @@ struct pfc_ctx {
     hipStream_t stream;
     int ncu;       // compute units of the device
     int persist;   // FP64 row pass as a persistent launch (PFC_PERSIST=0 turns it off)
+    // FP64 column pass of non-refresh steps per field (k_col_step_f), which
+    // writes the new state to the other of two state buffers: Nh2, allocated
+    // only for this (PFC_COLSPLIT=0 turns it off)
+    int colsplit;
+    void* Nh2;
     int generic;   // shape or U outside the tuned kernels: the generic path (pfc_generic.hip)
     void* gen;     // its plans and tables
     int have_fields, have_box;
@@ -173,6 +178,10 @@ template <typename T>
 struct Launch {
     static void row_step_any(pfc_ctx* c, int t, bool ref);
     static void col_step_any(pfc_ctx* c, bool ref);
+    // per-field column pass of a non-refresh step: state read from src,
+    // written to dst (both U * nkn elements); false if not built for Ny
+    static bool col_step_split_any(pfc_ctx* c, const void* src, void* dst);
+    static bool col_split_built(int Ny);
     static void row_c2r_any(pfc_ctx* c, const void* src, int sS, void* dst, void* K, int nbuf);
     static void row_r2c_any(pfc_ctx* c, const void* src, void* dst, int sD, int nbuf);
     static void col_fwd_any(pfc_ctx* c, const void* src, int sS, void* dst, int nbuf, double scale);

@@ -121,6 +121,46 @@ struct LaunchImpl {
         }
     }
 
+    // per-field column pass (FP64, 2^k columns)
+    static constexpr bool split_built(int ny) {
+        return sizeof(T) == 8 && size_built(ny) && (ny & (ny - 1)) == 0;
+    }
+    template <int NY, int U>
+    static void col_split(pfc_ctx* c, const void* src, void* dst) {
+        if constexpr (split_built(NY)) {
+            constexpr ColCfg g = col_step_cfg(NY);
+            const int H = c->Nx / 2;
+            hipLaunchKernelGGL((k_col_step_f<T, NY, g.tpb, g.tc, U>), dim3((H / g.tc + 1) * U),
+                               dim3(g.tpb * g.tc), 0, c->stream, (const C*)c->J, (const C*)src,
+                               (C*)dst, (C*)c->I, c->nk, c->nkn, H, c->sI, c->sJ,
+                               (const C*)c->twy, (const T*)c->ktab, c->dprm);
+        }
+    }
+    template <int NY>
+    static void col_split_n(pfc_ctx* c, const void* src, void* dst) {
+        switch (c->U) {
+            case 1: col_split<NY, 1>(c, src, dst); break;
+            case 2: col_split<NY, 2>(c, src, dst); break;
+            case 3: col_split<NY, 3>(c, src, dst); break;
+            case 4: col_split<NY, 4>(c, src, dst); break;
+        }
+    }
+    static bool col_step_split_any(pfc_ctx* c, const void* src, void* dst) {
+        if (!split_built(c->Ny)) return false;
+        switch (c->Ny) {
+            case 64: col_split_n<64>(c, src, dst); break;
+            case 128: col_split_n<128>(c, src, dst); break;
+            case 256: col_split_n<256>(c, src, dst); break;
+            case 512: col_split_n<512>(c, src, dst); break;
+            case 1024: col_split_n<1024>(c, src, dst); break;
+            case 2048: col_split_n<2048>(c, src, dst); break;
+            case 4096: col_split_n<4096>(c, src, dst); break;
+            case 8192: col_split_n<8192>(c, src, dst); break;
+            default: return false;
+        }
+        return true;
+    }
+
     template <int NX>
     static void row_c2r(pfc_ctx* c, const void* src, int sS, void* dst, void* K, int nbuf) {
         if constexpr (size_built(NX)) {
@@ -277,6 +317,12 @@ template <typename T>
 void Launch<T>::row_step_any(pfc_ctx* c, int t, bool ref) { LaunchImpl<T>::row_step_any(c, t, ref); }
 template <typename T>
 void Launch<T>::col_step_any(pfc_ctx* c, bool ref) { LaunchImpl<T>::col_step_any(c, ref); }
+template <typename T>
+bool Launch<T>::col_step_split_any(pfc_ctx* c, const void* src, void* dst) {
+    return LaunchImpl<T>::col_step_split_any(c, src, dst);
+}
+template <typename T>
+bool Launch<T>::col_split_built(int Ny) { return LaunchImpl<T>::split_built(Ny); }
 template <typename T>
 void Launch<T>::row_c2r_any(pfc_ctx* c, const void* src, int sS, void* dst, void* K, int nbuf) {
     LaunchImpl<T>::row_c2r_any(c, src, sS, dst, K, nbuf);
