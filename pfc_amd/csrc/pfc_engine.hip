@@ -255,9 +255,16 @@ int pfc_create(pfc_ctx** out, const pfc_params* p, int Nx, int Ny, int precision
     {
         const char* e = getenv("PFC_PERSIST");
         c->persist = !(e && e[0] == '0');
+        // The per-field column pass where it measured faster (4096^2 and
+        // 8192^2 with three fields: FP64 column pass 399 -> 363 us and
+        // 2.40 -> 1.71 ms, FP32 at 8192 831 -> 750 us; FP32 columns of 4096,
+        // whose in-place pass already runs two blocks per CU, 162 -> 180 us);
+        // PFC_COLSPLIT=1 takes it wherever it is built, =0 nowhere.
         e = getenv("PFC_COLSPLIT");
-        c->colsplit = !(e && e[0] == '0') && precision == PFC_FP64 && !generic &&
-                      Launch<double>::col_split_built(Ny);
+        const bool built = !generic && (precision == PFC_FP64 ? Launch<double>::col_split_built(Ny)
+                                                              : Launch<float>::col_split_built(Ny));
+        const bool pays = p->U >= 2 && Ny >= (precision == PFC_FP64 ? 4096 : 8192);
+        c->colsplit = built && (e && e[0] ? e[0] == '1' : pays);
     }
     c->nO = 2 * c->U + c->U * (c->U - 1) / 2;
     c->nkn = (size_t)(Nx / 2 + 1) * Ny;   // half spectrum, kx = 0..Nx/2
@@ -298,7 +305,8 @@ int pfc_create(pfc_ctx** out, const pfc_params* p, int Nx, int Ny, int precision
     rc |= dalloc((void**)&c->sums, (2 + c->U) * sizeof(double));
     rc |= dalloc((void**)&c->nan_flag, sizeof(int));
     rc |= dalloc((void**)&c->dprm, sizeof(DevParams));
-    rc |= dalloc((void**)&c->n0, 2 * c->U * sizeof(double));
+    // the k = 0 state and its alternate of the per-field column pass
+    rc |= dalloc((void**)&c->n0, 4 * c->U * sizeof(double));
 #if PFC_PROBE
     rc |= dalloc((void**)&c->hp.probe, 2 * 65536 * sizeof(unsigned long long));
 #endif
@@ -431,7 +439,10 @@ int pfc_step(pfc_ctx* c, int t0, int nsteps) {
         // refresh steps; a step without a partner takes the in-place pass.
         const bool split = c->Nh2 && (alt || (!ref && i + 1 < nsteps && (t + 1) % 100 != 0));
         if (split) {
-            Launch<double>::col_step_split_any(c, alt ? c->Nh2 : c->Nh, alt ? c->Nh : c->Nh2);
+            if (c->prec == PFC_FP64)
+                Launch<double>::col_step_split_any(c, alt ? c->Nh2 : c->Nh, alt ? c->Nh : c->Nh2);
+            else
+                Launch<float>::col_step_split_any(c, alt ? c->Nh2 : c->Nh, alt ? c->Nh : c->Nh2);
             alt = !alt;
         } else if (c->prec == PFC_FP64) {
             Launch<double>::col_step_any(c, ref);

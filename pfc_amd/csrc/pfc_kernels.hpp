@@ -1099,7 +1099,8 @@ template <typename T, int NY, int TPB>
 constexpr int colf_tw_entries() {
     return BlockFFT<T, NY, TPB, -1, 1, 1>::kTab;
 }
-// state loads of the per-field pass in batches of QB k points (U loads each)
+// state loads of the per-field pass in batches of QB k points (U loads each;
+// 1 and 4 measured slower or spilled)
 #ifndef PFC_COLF_QB
 #define PFC_COLF_QB 2
 #endif
@@ -1117,7 +1118,8 @@ __global__ __launch_bounds__(TPB* TC) PFC_WAVES((colf_waves<T, NY>())) void k_co
     const cpx<T>* __restrict__ J, const cpx<T>* __restrict__ Nsrc, cpx<T>* __restrict__ Ndst,
     cpx<T>* __restrict__ I, size_t fstride, size_t nstride, int H, int sI, int sJ,
     const cpx<T>* __restrict__ twy_, const T* __restrict__ ktab_,
-    const DevParams* __restrict__ prm) {
+    const DevParams* __restrict__ prm, const double* __restrict__ n0src,
+    double* __restrict__ n0dst) {
     constexpr int P = NY / TPB;
     constexpr unsigned E = sizeof(cpx<T>);
     using FFTF = BlockFFT<T, NY, TPB, -1, 1, 1, step_padc<T>()>;
@@ -1171,6 +1173,8 @@ __global__ __launch_bounds__(TPB* TC) PFC_WAVES((colf_waves<T, NY>())) void k_co
     if (jb.live) {
         const double kxd = kx * prm->dkx;
         const double kx2 = kxd * kxd;
+        // state loads in batches of QB k points, U fields each (issuing each
+        // batch one batch ahead measured 3 % slower at 4096^2)
         constexpr int QB = (P % PFC_COLF_QB == 0) ? PFC_COLF_QB : 1;
         const GBuf nd = gbuf(Ndst + f * nstride);
 #pragma unroll
@@ -1190,8 +1194,35 @@ __global__ __launch_bounds__(TPB* TC) PFC_WAVES((colf_waves<T, NY>())) void k_co
                 const int j = t + q * TPB;
                 const double ky = ky_of(j, NY, prm->dky);
                 const T k2 = (T)(kx2 + ky * ky);
-                M[q] = kupdate_one<U, T>(prm, f, k2, gx * gyb[i], M[q], Nb[i]);
+                bool k0 = false;
+                if constexpr (sizeof(T) == 4) k0 = (jb.mode == 1 && j == 0);
+                if (k0) {
+                    // k = 0 in FP64 from the FP64 states of all fields (n0src),
+                    // as the in-place pass; field f's result to n0dst
+                    cpx<double> N0[U];
+#pragma unroll
+                    for (int v = 0; v < U; v++) N0[v] = {n0src[2 * v], n0src[2 * v + 1]};
+                    const cpx<double> r = kupdate_one<U, double>(
+                        prm, f, 0.0, 1.0, cpx<double>{(double)M[q].x, (double)M[q].y}, N0);
+                    n0dst[2 * f] = r.x;
+                    n0dst[2 * f + 1] = r.y;
+                    M[q] = {(T)r.x, (T)r.y};
+                } else {
+                    M[q] = kupdate_one<U, T>(prm, f, k2, gx * gyb[i], M[q], Nb[i]);
+                }
                 gstore<T, PFC_COL_AUX>(nd, vn, q * TPB * E, M[q]);
+            }
+        }
+    }
+    if constexpr (sizeof(T) == 4) {
+        // kx = 0 and kx = H: the stored state replaced by its Hermitian part
+        if (cb == 0 || cb == H / TC) {
+            cpx<T>(&MM)[1][P] = *reinterpret_cast<cpx<T>(*)[1][P]>(&M);
+            hermitian_cols<T, NY, TPB, 1>(MM, L, t, jb.live ? jb.mode : 0);
+            if (jb.live && jb.mode != 0) {
+                const GBuf nd = gbuf(Ndst + f * nstride);
+#pragma unroll
+                for (int q = 0; q < P; q++) gstore<T>(nd, vn, q * TPB * E, M[q]);
             }
         }
     }

@@ -139,9 +139,10 @@ struct pfc_ctx {
     hipStream_t stream;
     int ncu;       // compute units of the device
     int persist;   // FP64 row pass as a persistent launch (PFC_PERSIST=0 turns it off)
-    // FP64 column pass of non-refresh steps per field (k_col_step_f), which
-    // writes the new state to the other of two state buffers: Nh2, allocated
-    // only for this (PFC_COLSPLIT=0 turns it off)
+    // column pass of non-refresh steps per field (k_col_step_f), which writes
+    // the new state to the other of two state buffers: Nh2, allocated only
+    // for this (PFC_COLSPLIT=0 turns it off); the mixed form's FP64 k = 0
+    // state alternates likewise between n0 and n0 + 2U
     int colsplit;
     void* Nh2;
     int generic;   // shape or U outside the tuned kernels: the generic path (pfc_generic.hip)

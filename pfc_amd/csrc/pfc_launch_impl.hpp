@@ -121,19 +121,23 @@ struct LaunchImpl {
         }
     }
 
-    // per-field column pass (FP64, 2^k columns)
+    // per-field column pass (2^k columns); the mixed form's FP64 k = 0
+    // state alternates between n0 and n0 + 2U with the spectral state
     static constexpr bool split_built(int ny) {
-        return sizeof(T) == 8 && size_built(ny) && (ny & (ny - 1)) == 0;
+        return size_built(ny) && (ny & (ny - 1)) == 0;
     }
     template <int NY, int U>
     static void col_split(pfc_ctx* c, const void* src, void* dst) {
         if constexpr (split_built(NY)) {
             constexpr ColCfg g = col_step_cfg(NY);
             const int H = c->Nx / 2;
+            const bool back = src == c->Nh2;
             hipLaunchKernelGGL((k_col_step_f<T, NY, g.tpb, g.tc, U>), dim3((H / g.tc + 1) * U),
                                dim3(g.tpb * g.tc), 0, c->stream, (const C*)c->J, (const C*)src,
                                (C*)dst, (C*)c->I, c->nk, c->nkn, H, c->sI, c->sJ,
-                               (const C*)c->twy, (const T*)c->ktab, c->dprm);
+                               (const C*)c->twy, (const T*)c->ktab, c->dprm,
+                               (const double*)(back ? c->n0 + 2 * U : c->n0),
+                               back ? c->n0 : c->n0 + 2 * U);
         }
     }
     template <int NY>
