@@ -18,8 +18,9 @@ the workload's 5000 steps).
 
 Prints one JSON line.  `roofline` is for the dominant kernel of the FP64 step
 (the longer of its row and column passes), timed with HIP events on the
-engine's stream over a second window of K steps; it also carries both passes'
-fractions and the whole step's fraction under the three-sweep byte model.
+engine's stream over a second window of plain (non-refresh) steps; it also
+carries both passes' fractions and the whole step's fraction under the
+three-sweep byte model.
 `cpu_baseline` times the CPU oracle on a bounded sample of the same workload;
 `batch` is the batch form on 96 systems of uneven sizes (2^k and 3*2^k sides
 from 256 to 2048, one or three fields), beside the same systems stepped one
@@ -87,11 +88,12 @@ def relax_window(g, t0, nsteps, tprint):
 
 
 def run_gpu(params, fields, Nx, Ny, dx, dy, precision, steps, warmup, tprint=TPRINT):
-    """Warmup, then two windows of `steps` steps, each starting at a multiple
-    of 100 so that it holds a refresh step and an evaluate (one per tprint
-    steps, as relax() runs them).  The first window is timed by the wall clock
-    alone (the figure reported); the second records HIP events around each
-    pass on the engine's stream (per-pass times for the roofline)."""
+    """Warmup, then a window of `steps` steps starting at a multiple of 100, so
+    that it holds a refresh step and an evaluate (one per tprint steps, as
+    relax() runs them), timed by the wall clock alone (the figure reported).
+    Then a second window of plain steps (no refresh, an even number, at most
+    98) records HIP events around each pass on the engine's stream: the mean
+    launch time of each pass's plain kernel, for the roofline."""
     from pfc_amd.engine import Engine
     g = Engine(params, Nx, Ny, dx, dy, precision)
     g.set_fields(fields)
@@ -104,10 +106,10 @@ def run_gpu(params, fields, Nx, Ny, dx, dy, precision, steps, warmup, tprint=TPR
     g.nan_step()
     wall = time.perf_counter() - t0
     s1 = -(-(s0 + steps) // 100) * 100
-    g.step(s0 + steps, s1 - s0 - steps)
+    g.step(s0 + steps, s1 - s0 + 1 - steps)     # through the refresh step s1
     g.nan_step()
     g.timing(True)
-    relax_window(g, s1, steps, tprint)
+    g.step(s1 + 1, max(2, min(steps, 98) // 2 * 2))
     ms, n = g.timing_read()
     g.timing(False)
     assert g.nan_step() == -1, "NaN in the benchmark run"
@@ -490,10 +492,15 @@ def main(argv=None):
     def kernel_roof(r, pname, esz):
         dom = "column" if r["col_ms"] >= r["row_ms"] else "row"
         ach = r["col_gbs"] if dom == "column" else r["row_gbs"]
+        # the engine's choice of column pass for plain steps (pfc_create):
+        # per field for three fields at Ny >= 4096 (FP64) or 8192 (mixed)
+        split = os.environ.get("PFC_COLSPLIT", "") != "0" and (
+            os.environ.get("PFC_COLSPLIT", "") == "1" or Ny >= (4096 if pname == "fp64" else 8192))
+        col_kernel = "k_col_step_f (column pass per field, " if split else "k_col_step (column pass, "
         return {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": ach / PEAK_HBM_GBS,
                 "traffic": traffic_from_profile(Nx, Ny, f"{pname}_{dom}_bytes_per_launch"),
-                "kernel": ("k_col_step (column pass, " if dom == "column" else
+                "kernel": (col_kernel if dom == "column" else
                            ("k_row_step_p (row pass, persistent, " if pname == "fp64" else
                             "k_row_step (row pass, ")) + pname + ")",
                 "algorithmic_bytes_per_launch": (col_bytes if dom == "column" else row_bytes)(
@@ -524,8 +531,12 @@ def main(argv=None):
                                f"reference's precision); the mixed FP32/FP64 form beside it",
                    "grid": [Nx, Ny], "fields": U, "parallelism": "1 GPU",
                    "timed_window": f"steps {d['window_start']}..{d['window_start'] + a.steps - 1} "
-                                   f"with {d['evaluates']} evaluate(s) (one per {TPRINT} steps, "
-                                   f"as relax() prints) and the t % 100 refresh steps"},
+                                   f"with {d['evaluates']} evaluate(s) (one before each step t "
+                                   f"with t % {TPRINT} == 0, as relax() prints) and the t % 100 "
+                                   f"refresh steps"
+                                   + (f"; a window of {a.steps} < {TPRINT} steps still holds one "
+                                      f"of each, {TPRINT / a.steps:.0f}x relax()'s rate"
+                                      if a.steps < TPRINT else "")},
         "fp64": {"value": d["gps"], "ms_per_step": d["ms_step"],
                  "column_pass_GBps": d["col_gbs"], "row_pass_GBps": d["row_gbs"],
                  "column_pass_ms": d["col_ms"], "row_pass_ms": d["row_ms"],

@@ -568,6 +568,10 @@ int pfc_batch_step(pfc_batch* b, int t0, int nsteps) {
         if (!c->have_fields || !c->have_box) return PFC_ESTATE;
         if (hipStreamSynchronize(c->stream) != hipSuccess) return PFC_EDEVICE;
     }
+    // the batch refreshes at every t % 100 step itself; after it a
+    // context's state is no longer the one its last evaluate computed
+    if (nsteps > 0)
+        for (pfc_ctx* c : b->ctx) c->nh_fresh = 0;
     b->thost = t0;
     if (hipMemcpyAsync(b->tdev, &b->thost, sizeof(int), hipMemcpyHostToDevice, b->s) != hipSuccess)
         return PFC_EDEVICE;

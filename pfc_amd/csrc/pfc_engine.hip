@@ -203,6 +203,7 @@ int materialise(pfc_ctx* c, bool refresh) {
         if (refresh) Launch<float>::col_fwd_any(c, c->K, c->sJ, c->Nh, c->U, c->hp.inv_NxNy);
     }
     HIPCHK(hipGetLastError());
+    if (refresh) c->nh_fresh = 1;
     return PFC_OK;
 }
 
@@ -408,6 +409,7 @@ int pfc_set_fields(pfc_ctx* c, const double* fields) {
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
     c->have_fields = 1;
+    c->nh_fresh = !c->generic;   // Nh = FFT2(m)/(Nx Ny), as a refresh computes it
     return PFC_OK;
 }
 
@@ -415,9 +417,14 @@ int pfc_step(pfc_ctx* c, int t0, int nsteps) {
     if (!c || nsteps < 0) return PFC_EINVAL;
     if (!c->have_fields || !c->have_box) return PFC_ESTATE;
     bool alt = false;   // the state is in Nh2 (between the two steps of a pair)
+    const bool fresh = c->nh_fresh;
+    if (nsteps > 0) c->nh_fresh = 0;
     for (int i = 0; i < nsteps; i++) {
         const int t = t0 + i;
-        const bool ref = (t % 100 == 0);
+        // the refresh of src/pfc-relax.c:1178-1184, unless an evaluate or
+        // pfc_set_fields has just computed that same state from the same
+        // densities (relax() evaluates before every printed step)
+        const bool ref = (t % 100 == 0) && !(i == 0 && fresh);
         hipEvent_t* ev = nullptr;
         if (c->timing) {
             if (ensure_events(c, c->nrec + 1) != PFC_OK) return PFC_EDEVICE;
@@ -458,6 +465,7 @@ int pfc_step(pfc_ctx* c, int t0, int nsteps) {
 int pfc_perturb(pfc_ctx* c, double rel, unsigned long long seed) {
     if (!c || !(rel >= 0.0)) return PFC_EINVAL;
     if (!c->have_fields) return PFC_ESTATE;
+    c->nh_fresh = 0;
     // real parts of the density's state (the generic path keeps the density
     // itself) and of Nh
     const size_t ni = c->generic ? c->U * c->nr : 2 * c->U * c->nk, nn = 2 * c->U * c->nkn;
@@ -527,6 +535,7 @@ int pfc_evaluate(pfc_ctx* c, double* fave, double* nave) {
         Launch<float>::row_c2r_any(c, c->O, c->sC, c->R, nullptr, c->nO);
         Launch<float>::energy_any(c);
     }
+    c->nh_fresh = !c->generic;   // k_col_eval stored the refreshed state
     hipLaunchKernelGGL((k_reduce_final<1024>), dim3(1), dim3(1024), 0, c->stream, c->partials,
                        kEnergyBlocks, 1 + c->U, c->sums, c->hp.inv_NxNy);
     HIPCHK(hipGetLastError());

@@ -145,6 +145,10 @@ struct pfc_ctx {
     // state alternates likewise between n0 and n0 + 2U
     int colsplit;
     void* Nh2;
+    // Nh was recomputed from the current densities (I) by an evaluate or
+    // pfc_set_fields and no step has run since: a t % 100 step then skips
+    // its refresh, which would recompute the same state (tuned path only)
+    int nh_fresh;
     int generic;   // shape or U outside the tuned kernels: the generic path (pfc_generic.hip)
     void* gen;     // its plans and tables
     int have_fields, have_box;
