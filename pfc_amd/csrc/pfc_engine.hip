@@ -552,14 +552,26 @@ int pfc_evaluate(pfc_ctx* c, double* fave, double* nave) {
 int pfc_evaluate_sums(pfc_ctx* c, double* fave, double* nave) {
     if (!c || !fave || !nave) return PFC_EINVAL;
     if (!c->have_fields || !c->have_box) return PFC_ESTATE;
-    // the densities and the refreshed state, as the full evaluate leaves them
-    const int rc = materialise(c, true);
-    if (rc) return rc;
-    if (c->generic) gen_energy_sums(c);
-    else if (c->prec == PFC_FP64) Launch<double>::energy_sums_any(c);
-    else Launch<float>::energy_sums_any(c);
+    // the refreshed state, as the full evaluate leaves it; the tuned path
+    // takes the local energy terms from the densities in registers during
+    // their row pass and writes no densities
+    int nloc = kEnergyBlocks;
+    if (c->generic) {
+        const int rc = materialise(c, true);
+        if (rc) return rc;
+        gen_energy_sums(c);
+    } else if (c->prec == PFC_FP64) {
+        nloc = Launch<double>::eval_local_any(c);
+        Launch<double>::col_fwd_any(c, c->K, c->sJ, c->Nh, c->U, c->hp.inv_NxNy);
+        Launch<double>::energy_spec_any(c);
+    } else {
+        nloc = Launch<float>::eval_local_any(c);
+        Launch<float>::col_fwd_any(c, c->K, c->sJ, c->Nh, c->U, c->hp.inv_NxNy);
+        Launch<float>::energy_spec_any(c);
+    }
+    if (!c->generic) c->nh_fresh = 1;
     hipLaunchKernelGGL((k_reduce_final<1024>), dim3(1), dim3(1024), 0, c->stream, c->partials,
-                       kEnergyBlocks, 1 + c->U, c->sums, c->hp.inv_NxNy);
+                       nloc, 1 + c->U, c->sums, c->hp.inv_NxNy);
     hipLaunchKernelGGL((k_reduce_final<1024>), dim3(1), dim3(1024), 0, c->stream,
                        c->partials + (size_t)kEnergyBlocks * 2 * (1 + c->U), kEnergyBlocks, 1,
                        c->sums + 1 + c->U, 1.0);

@@ -1633,6 +1633,100 @@ __global__ __launch_bounds__(BS) void k_energy_local(const T* __restrict__ n, si
 // Re(N_u conj N_v) with the real-space path's multipliers (eval_mult).  Nh
 // holds nkn = (Nx/2 + 1) Ny entries per field, kx major.  Per-block
 // compensated partials of the one sum.
+// The densities' row pass of an evaluate without the energy density: per
+// row group, the C2R of every field's row state (the densities, kept in
+// registers), their R2C into K for the refresh N = FFT(n)/NxNy
+// (src/pfc-relax.c:1126-1128), and the local terms of the free energy with
+// each field's sum, as k_energy_local computes them, without writing the
+// densities to memory.  A grid of at most kEnergyBlocks blocks walks the row
+// groups (XCD-contiguous, persist_row_group); each block writes one set of
+// partial sums.
+template <typename T, int NX, int TPB, int TR, int U>
+__global__ __launch_bounds__(TPB* TR) void k_row_c2r_eloc(
+    const cpx<T>* __restrict__ I, cpx<T>* __restrict__ K, size_t fstride, int sI, int sK,
+    const cpx<T>* __restrict__ twx_, const DevParams* __restrict__ p,
+    double* __restrict__ partials, int ngroups) {
+    constexpr int H = NX / 2;
+    constexpr int P = H / TPB;
+    constexpr int BS = TPB * TR;
+    constexpr unsigned E = sizeof(cpx<T>);
+    using FFTF = BlockFFT<T, H, TPB, -1, 2, TR>;
+    using FFTI = BlockFFT<T, H, TPB, +1, 2, TR>;
+    __shared__ cpx<T> lds[H * TR];
+    const GBuf twx = gbuf(twx_);
+    const int tid = threadIdx.x;
+    const int r = tid % TR;
+    const int t = tid / TR;
+    cpx<T>* L = lds + r;
+    const unsigned istep = lay_row_step(sI, TPB, E), kstep = lay_row_step(sK, TPB, E);
+    const double _3 = 1.0 / 3.0;
+    KSum acc[1 + U];
+#pragma unroll
+    for (int i = 0; i <= U; i++) acc[i] = {0.0, 0.0};
+    for (int it = 0;; it++) {
+        const int g = persist_row_group(blockIdx.x, gridDim.x, it, ngroups);
+        if (g >= ngroups) break;
+        const int y = g * TR + r;
+        const unsigned ioff = lay_row_off(sI, H, t, y, E), koff = lay_row_off(sK, H, t, y, E);
+        cpx<T> m[U][P];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const GBuf src = gbuf(I + u * fstride);
+#pragma unroll
+            for (int q = 0; q < P; q++) m[u][q] = gload<T>(src, ioff, q * istep);
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const cpx<T> wt = gload<T>(twx, (unsigned)t * E, 0u);
+            c2r_pre<T, H, TPB, TR>(m[u], L, wt, t);
+            FFTI::run(m[u], L, twx, t);
+            cpx<T> w[P];
+#pragma unroll
+            for (int q = 0; q < P; q++) w[q] = m[u][q];
+            FFTF::run(w, L, twx, t);
+            r2c_post<T, H, TPB, TR>(w, L, wt, t);
+            const GBuf kb = gbuf(K + u * fstride);
+#pragma unroll
+            for (int q = 0; q < P; q++) gstore<T>(kb, koff, q * kstep, w[q]);
+        }
+        // element q of a lane holds the densities at x = 2(t + q TPB) (re)
+        // and x + 1 (im); the same expression as k_energy_local per point
+#pragma unroll
+        for (int q = 0; q < P; q++) {
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                double nv[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) nv[u] = (double)(h ? m[u][q].y : m[u][q].x);
+                double fij = 0.0;
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const double nu = nv[u];
+                    const double nunu = nu * nu;
+                    fij += p->lambda[u] * nu + 0.5 * p->alpha[u][u] * nunu +
+                           _3 * p->gamma[u][u] * nu * nunu + 0.25 * p->delta[u] * nunu * nunu;
+#pragma unroll
+                    for (int v = u + 1; v < U; v++) {
+                        const double nvv = nv[v];
+                        fij += p->alpha[u][v] * nu * nvv +
+                               0.5 * p->gamma[u][v] * (nunu * nvv + nu * nvv * nvv);
+                    }
+                    kadd(acc[1 + u], nu);
+                }
+                kadd(acc[0], fij);
+            }
+        }
+    }
+    block_ksum<1 + U, BS>(acc);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i <= U; i++) {
+            partials[(size_t)blockIdx.x * 2 * (1 + U) + 2 * i] = acc[i].s;
+            partials[(size_t)blockIdx.x * 2 * (1 + U) + 2 * i + 1] = acc[i].c;
+        }
+    }
+}
+
 template <typename T, int U, int BS>
 __global__ __launch_bounds__(BS) void k_energy_spec(const cpx<T>* __restrict__ Nh, size_t nkn,
                                                     int Nx, int Ny, double* __restrict__ partials,

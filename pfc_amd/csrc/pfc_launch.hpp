@@ -194,6 +194,10 @@ struct Launch {
     static void col_eval_any(pfc_ctx* c);
     static void energy_any(pfc_ctx* c);
     static void energy_sums_any(pfc_ctx* c);   // after the refresh: Nh and mreal current
+    // the densities' row pass of pfc_evaluate_sums: K and the local energy
+    // sums, without the densities (returns the partial-sum blocks)
+    static int eval_local_any(pfc_ctx* c);
+    static void energy_spec_any(pfc_ctx* c);   // the spectral sums from Nh
 };
 
 }  // namespace pfc

@@ -302,6 +302,63 @@ struct LaunchImpl {
             case 4: energy<4>(c); break;
         }
     }
+    // evaluate without the energy density: the densities' row pass with the
+    // local energy sums (k_row_c2r_eloc, K for the refresh); returns its
+    // number of partial-sum blocks
+    template <int NX, int U>
+    static int eval_local(pfc_ctx* c) {
+        if constexpr (size_built(NX)) {
+            constexpr RowCfg g = row_cfg<T>(NX);
+            const int ng = c->Ny / g.tr;
+            int nb = ng < kEnergyBlocks ? ng : kEnergyBlocks;
+            if (nb >= 8) nb -= nb % 8;
+            hipLaunchKernelGGL((k_row_c2r_eloc<T, NX, g.tpb, g.tr, U>), dim3(nb), dim3(g.tpb * g.tr),
+                               0, c->stream, (const C*)c->I, (C*)c->K, c->nk, c->sI, c->sJ,
+                               (const C*)c->twx, c->dprm, c->partials, ng);
+            return nb;
+        }
+        return 0;
+    }
+    template <int NX>
+    static int eval_local_n(pfc_ctx* c) {
+        switch (c->U) {
+            case 1: return eval_local<NX, 1>(c);
+            case 2: return eval_local<NX, 2>(c);
+            case 3: return eval_local<NX, 3>(c);
+            case 4: return eval_local<NX, 4>(c);
+        }
+        return 0;
+    }
+    static int eval_local_any(pfc_ctx* c) {
+        switch (c->Nx) {
+            case 64: return eval_local_n<64>(c);
+            case 128: return eval_local_n<128>(c);
+            case 256: return eval_local_n<256>(c);
+            case 384: return eval_local_n<384>(c);
+            case 512: return eval_local_n<512>(c);
+            case 768: return eval_local_n<768>(c);
+            case 1024: return eval_local_n<1024>(c);
+            case 1536: return eval_local_n<1536>(c);
+            case 2048: return eval_local_n<2048>(c);
+            case 4096: return eval_local_n<4096>(c);
+            case 8192: return eval_local_n<8192>(c);
+        }
+        return 0;
+    }
+    template <int U>
+    static void energy_spec(pfc_ctx* c) {
+        hipLaunchKernelGGL((k_energy_spec<T, U, 256>), dim3(kEnergyBlocks), dim3(256), 0, c->stream,
+                           (const C*)c->Nh, c->nkn, c->Nx, c->Ny,
+                           c->partials + (size_t)kEnergyBlocks * 2 * (1 + U), c->dprm);
+    }
+    static void energy_spec_any(pfc_ctx* c) {
+        switch (c->U) {
+            case 1: energy_spec<1>(c); break;
+            case 2: energy_spec<2>(c); break;
+            case 3: energy_spec<3>(c); break;
+            case 4: energy_spec<4>(c); break;
+        }
+    }
     template <int U>
     static void energy_sums(pfc_ctx* c) {
         launch_energy_sums<T, U>((const T*)c->mreal, (const C*)c->Nh, c->nr, c->nkn, c->Nx, c->Ny,
@@ -349,5 +406,9 @@ template <typename T>
 void Launch<T>::energy_any(pfc_ctx* c) { LaunchImpl<T>::energy_any(c); }
 template <typename T>
 void Launch<T>::energy_sums_any(pfc_ctx* c) { LaunchImpl<T>::energy_sums_any(c); }
+template <typename T>
+int Launch<T>::eval_local_any(pfc_ctx* c) { return LaunchImpl<T>::eval_local_any(c); }
+template <typename T>
+void Launch<T>::energy_spec_any(pfc_ctx* c) { LaunchImpl<T>::energy_spec_any(c); }
 
 }  // namespace pfc
