@@ -153,12 +153,16 @@ __device__ __forceinline__ double ky_of(int j, int Ny, double dky) {
 }
 
 // Row-spectral layouts.  A buffer of H columns by Ny rows is stored in
-// y-blocks of YB = 2^s rows: element (kx, y) at (y >> s) Q + kx P + (y mod YB),
-// with column pitch P and block pitch Q.
-// s = log2(Ny) is the column-major layout kx P + y (whole columns
+// y-blocks of YB = 2^s rows, and within a y-block in groups of XB = 2^x
+// neighbouring columns: element (kx, y) at
+//   (y >> s) Q + (kx >> x) P + (y mod YB) XB + (kx mod XB),
+// with group pitch P and block pitch Q.  A layout code is s + 32 x.
+// s = log2(Ny), x = 0 is the column-major layout kx P + y (whole columns
 // contiguous); s = log2(TR) makes the TR rows of a row-pass block one
-// contiguous run of TR elements per kx.
-// Long strides are padded by PFC_CPAD bytes: P = YB (+ pad when YB >= 1024),
+// contiguous run of TR elements per kx; x = 1 interleaves the columns in
+// pairs, so that the row pass's TR rows of two neighbouring kx are one run
+// of 2 TR elements while a column pair stays contiguous.
+// Long strides are padded by PFC_CPAD bytes: P = YB XB (+ pad when YB >= 1024),
 // Q = H YB (+ pad when YB < 1024).  Unpadded, the lanes of the pass that cuts
 // across the layout (consecutive kx in the row pass on the column-major
 // layout) sit a power of two apart (64 KiB for 4096 FP64 rows) and fall into
@@ -173,31 +177,40 @@ __host__ __device__ constexpr int ilog2c(int n) {
     while ((1 << s) < n) s++;
     return s;
 }
-__host__ __device__ constexpr unsigned lay_pitch(int s, unsigned E) {
-    return (1u << s) + (s >= 10 ? (unsigned)PFC_CPAD / E : 0u);
+__host__ __device__ constexpr int lay_s(int code) { return code & 31; }
+__host__ __device__ constexpr int lay_x(int code) { return code >> 5; }
+__host__ __device__ constexpr unsigned lay_pitch(int code, unsigned E) {
+    return (1u << (lay_s(code) + lay_x(code))) + (lay_s(code) >= 10 ? (unsigned)PFC_CPAD / E : 0u);
 }
-__host__ __device__ constexpr unsigned lay_bpitch(int s, int H, unsigned E) {
-    return (unsigned)H * (1u << s) + (s < 10 ? (unsigned)PFC_CPAD / E : 0u);
+__host__ __device__ constexpr unsigned lay_bpitch(int code, int H, unsigned E) {
+    return (unsigned)H * (1u << lay_s(code)) + (lay_s(code) < 10 ? (unsigned)PFC_CPAD / E : 0u);
 }
-// Row-pass lane (kx = t + q TPB, row y): byte offset of q = 0 and the step in q.
-__device__ __forceinline__ unsigned lay_row_off(int s, int H, int t, int y, unsigned E) {
-    return ((unsigned)(y >> s) * lay_bpitch(s, H, E) + (unsigned)t * lay_pitch(s, E) +
-            (unsigned)(y & ((1 << s) - 1))) * E;
+// Row-pass lane (kx = t + q TPB, row y): byte offset of q = 0 and the step in
+// q (TPB a multiple of XB).
+__device__ __forceinline__ unsigned lay_row_off(int code, int H, int t, int y, unsigned E) {
+    const int s = lay_s(code), x = lay_x(code);
+    return ((unsigned)(y >> s) * lay_bpitch(code, H, E) + (unsigned)(t >> x) * lay_pitch(code, E) +
+            ((unsigned)(y & ((1 << s) - 1)) << x) + (unsigned)(t & ((1 << x) - 1))) * E;
 }
-__device__ __forceinline__ unsigned lay_row_step(int s, int TPB, unsigned E) {
-    return ((unsigned)TPB * lay_pitch(s, E)) * E;
+__device__ __forceinline__ unsigned lay_row_step(int code, int TPB, unsigned E) {
+    return ((unsigned)(TPB >> lay_x(code)) * lay_pitch(code, E)) * E;
 }
 // Column-pass lane (column kx, y = t + q TPB); needs TPB >= YB or YB >= Ny.
-__device__ __forceinline__ unsigned lay_col_off(int s, int H, int kx, int t, unsigned E) {
-    return ((unsigned)(t >> s) * lay_bpitch(s, H, E) + (unsigned)kx * lay_pitch(s, E) +
-            (unsigned)(t & ((1 << s) - 1))) * E;
+__device__ __forceinline__ unsigned lay_col_off(int code, int H, int kx, int t, unsigned E) {
+    const int s = lay_s(code), x = lay_x(code);
+    return ((unsigned)(t >> s) * lay_bpitch(code, H, E) + (unsigned)(kx >> x) * lay_pitch(code, E) +
+            ((unsigned)(t & ((1 << s) - 1)) << x) + (unsigned)(kx & ((1 << x) - 1))) * E;
 }
-__device__ __forceinline__ unsigned lay_col_step(int s, int H, int TPB, unsigned E) {
-    return (TPB >= (1 << s)) ? (unsigned)(TPB >> s) * lay_bpitch(s, H, E) * E : (unsigned)TPB * E;
+__device__ __forceinline__ unsigned lay_col_step(int code, int H, int TPB, unsigned E) {
+    const int s = lay_s(code), x = lay_x(code);
+    return (TPB >= (1 << s)) ? (unsigned)(TPB >> s) * lay_bpitch(code, H, E) * E
+                             : ((unsigned)TPB << x) * E;
 }
-// Elements per field of a buffer in layout s (one past the last element).
-inline size_t lay_elems(int s, int H, int Ny, unsigned E) {
-    return (size_t)(((Ny + (1 << s) - 1) >> s) - 1) * lay_bpitch(s, H, E) + (size_t)H * lay_pitch(s, E);
+// Elements per field of a buffer in layout code (one past the last element).
+inline size_t lay_elems(int code, int H, int Ny, unsigned E) {
+    const int s = lay_s(code), x = lay_x(code);
+    return (size_t)(((Ny + (1 << s) - 1) >> s) - 1) * lay_bpitch(code, H, E) +
+           (size_t)(H >> x) * lay_pitch(code, E);
 }
 
 // ------------------------------------------------ elementary functions
@@ -1040,13 +1053,16 @@ __global__ __launch_bounds__(TPB* TC) PFC_WAVES(col_waves<T>()) void k_col_step(
     constexpr bool LGY = col_lgy<T, NY, TPB, TC>();
     __shared__ cpx<T> lds[col_lds_len<T, NY, TPB>() * TC];
     __shared__ __attribute__((aligned(16))) cpx<T> ltw[col_tw_entries<T, NY, TPB, TC, LTAB>()];
+    // with J in column pairs, neighbouring column blocks on one XCD (they
+    // read the two halves of the same lines)
+    const int cb = lay_x(sJ) == 1 ? xcd_remap_any(blockIdx.x, gridDim.x) : (int)blockIdx.x;
     if constexpr (LGY) {
         __shared__ __attribute__((aligned(16))) T lgy[NY];   // exp(-ky^2/sigma^2)
-        col_step_body<T, NY, TPB, TC, U, REFRESH, LTAB, LGY>(lds, ltw, lgy, blockIdx.x, J, K, Nh, I,
+        col_step_body<T, NY, TPB, TC, U, REFRESH, LTAB, LGY>(lds, ltw, lgy, cb, J, K, Nh, I,
                                                              fstride, nstride, H, sI, sJ, twy_,
                                                              ktab_, prm, n0);
     } else {
-        col_step_body<T, NY, TPB, TC, U, REFRESH, LTAB, LGY>(lds, ltw, nullptr, blockIdx.x, J, K, Nh,
+        col_step_body<T, NY, TPB, TC, U, REFRESH, LTAB, LGY>(lds, ltw, nullptr, cb, J, K, Nh,
                                                              I, fstride, nstride, H, sI, sJ, twy_,
                                                              ktab_, prm, n0);
     }
@@ -1131,7 +1147,19 @@ __global__ __launch_bounds__(TPB* TC) PFC_WAVES((colf_waves<T, NY>())) void k_co
     __shared__ cpx<T> lds[col_lds_len<T, NY, TPB>() * TC];
     __shared__ __attribute__((aligned(16))) cpx<T> ltw[NTAB];
     const int item = xcd_remap_any(blockIdx.x, gridDim.x);
-    const int cb = item / U, f = item % U;
+    // items: (column block, field), field fastest; with J in column pairs
+    // (layout x = 1, one column per block) the two columns of a pair are
+    // consecutive items for each field, so that the halves of J's lines
+    // that the two blocks read are fetched once into their XCD's L2
+    int cb, f;
+    if (TC == 1 && lay_x(sJ) == 1 && item < H * U) {
+        const int rem = item % (2 * U);
+        cb = 2 * (item / (2 * U)) + (rem & 1);
+        f = rem >> 1;
+    } else {
+        cb = item / U;
+        f = item % U;
+    }
     const int tid = threadIdx.x;
     const int c = tid / TPB;
     const int t = tid % TPB;
