@@ -277,18 +277,25 @@ int pfc_create(pfc_ctx** out, const pfc_params* p, int Nx, int Ny, int precision
     c->sJ = ilog2(precision == PFC_FP64 ? row_block<double>(Nx, Ny, PFC_J_BLOCKED)
                                         : row_block<float>(Nx, Ny, PFC_J_BLOCKED));
     {
-        // J and K with their columns in pairs (layout x = 1): the row pass
-        // writes runs of two columns' TR rows, twice as long as column-major
-        // ones, and the column pass reads half lines (measured at 4096^2 and
-        // 8192^2 with three fields: FP64 row pass 355 -> 328 us and 1.98 ->
-        // 1.69 ms, column pass 378 -> 390 us and unchanged; FP32 at 8192 row
-        // 0.99 -> 0.84 ms, column 0.76 -> 0.78 ms; FP32 at 4096 the column
-        // pass lost more than the row pass gained).  I stays column-major:
-        // with the column pass writing half lines it took twice as long.
-        // PFC_XPAIR=J takes it for every tuned shape, =0 for none.
+        // J and K with their columns in groups of 2^x (layout x): the row
+        // pass writes runs of 2^x columns' TR rows, 2^x times as long as
+        // column-major ones, and the column pass reads parts of lines that
+        // its neighbouring column blocks read too.  Measured at 4096^2 and
+        // 8192^2 with three fields: FP64 pairs, row pass 355 -> 328 us and
+        // 1.98 -> 1.69 ms, column pass 378 -> 390 us and unchanged (groups of
+        // 4: column pass slower, step 3.46 -> 3.57 ms at 8192^2); FP32 at
+        // 8192 groups of 4, row 0.98 -> 0.73 ms, column 0.76 -> 0.83 ms (step
+        // 1.75 -> 1.57 ms; pairs 1.64 ms); FP32 at 4096 the column pass lost
+        // more than the row pass gained.  I stays column-major: with the
+        // column pass writing half lines it took twice as long.
+        // PFC_XPAIR=J takes pairs for every tuned shape, =4 groups of 4, =0
+        // neither.
         const char* e = getenv("PFC_XPAIR");
-        const bool big = precision == PFC_FP64 ? (Nx >= 4096 && Ny >= 4096) : (Nx >= 8192 && Ny >= 8192);
-        if (!generic && (e && e[0] ? e[0] == 'J' : big)) c->sJ |= 32;
+        int x = 0;
+        if (e && e[0]) x = e[0] == 'J' ? 1 : e[0] == '4' ? 2 : 0;
+        else if (precision == PFC_FP64) x = (Nx >= 4096 && Ny >= 4096) ? 1 : 0;
+        else x = (Nx >= 8192 && Ny >= 8192) ? 2 : 0;
+        if (!generic) c->sJ |= 32 * x;
     }
     // field stride of I, J, K, O: the largest of their layouts (padded strides)
     c->nk = std::max({lay_elems(c->sC, Nx / 2, Ny, c->esz), lay_elems(c->sI, Nx / 2, Ny, c->esz),

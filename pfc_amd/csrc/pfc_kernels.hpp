@@ -1053,9 +1053,9 @@ __global__ __launch_bounds__(TPB* TC) PFC_WAVES(col_waves<T>()) void k_col_step(
     constexpr bool LGY = col_lgy<T, NY, TPB, TC>();
     __shared__ cpx<T> lds[col_lds_len<T, NY, TPB>() * TC];
     __shared__ __attribute__((aligned(16))) cpx<T> ltw[col_tw_entries<T, NY, TPB, TC, LTAB>()];
-    // with J in column pairs, neighbouring column blocks on one XCD (they
-    // read the two halves of the same lines)
-    const int cb = lay_x(sJ) == 1 ? xcd_remap_any(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    // with J in column groups, neighbouring column blocks on one XCD (they
+    // read parts of the same lines)
+    const int cb = lay_x(sJ) >= 1 ? xcd_remap_any(blockIdx.x, gridDim.x) : (int)blockIdx.x;
     if constexpr (LGY) {
         __shared__ __attribute__((aligned(16))) T lgy[NY];   // exp(-ky^2/sigma^2)
         col_step_body<T, NY, TPB, TC, U, REFRESH, LTAB, LGY>(lds, ltw, lgy, cb, J, K, Nh, I,
@@ -1152,10 +1152,11 @@ __global__ __launch_bounds__(TPB* TC) PFC_WAVES((colf_waves<T, NY>())) void k_co
     // consecutive items for each field, so that the halves of J's lines
     // that the two blocks read are fetched once into their XCD's L2
     int cb, f;
-    if (TC == 1 && lay_x(sJ) == 1 && item < H * U) {
-        const int rem = item % (2 * U);
-        cb = 2 * (item / (2 * U)) + (rem & 1);
-        f = rem >> 1;
+    const int xb = 1 << lay_x(sJ);
+    if (TC == 1 && xb > 1 && item < H * U) {
+        const int rem = item % (xb * U);
+        cb = xb * (item / (xb * U)) + rem % xb;
+        f = rem / xb;
     } else {
         cb = item / U;
         f = item % U;
