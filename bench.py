@@ -73,16 +73,18 @@ def relax_window(g, t0, nsteps, tprint):
     """Steps t0 .. t0 + nsteps - 1 as relax() runs them (src/pfc-relax.c:
     1292-1312): an evaluate (the free energy and means that print_output
     logs) before every step t with t % tprint == 0, and each step t with
-    t % 100 == 0 refreshing the spectral state inside pfc_step.  hetero.in
-    writes no energy density file, so every evaluate is pfc_evaluate_sums, as
-    pfc-relax runs it.  Returns the number of evaluates."""
+    t % 100 == 0 refreshing the spectral state.  hetero.in writes no energy
+    density file, so each print is pfc_step_evaluate (the evaluate's sums
+    taken by the first step after it), as pfc-relax runs it.  Returns the
+    number of evaluates."""
     t, end, nev = t0, t0 + nsteps, 0
     while t < end:
-        if t % tprint == 0:
-            g.evaluate(density=False)
-            nev += 1
         n = min(end, (t // tprint + 1) * tprint) - t
-        g.step(t, n)
+        if t % tprint == 0:
+            g.step_evaluate(t, n)
+            nev += 1
+        else:
+            g.step(t, n)
         t += n
     return nev
 

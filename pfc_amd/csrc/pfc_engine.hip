@@ -134,6 +134,8 @@ void free_all(pfc_ctx* c) {
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     free(c->hktab);
+    if (c->hsums) (void)hipHostFree(c->hsums);
+    if (c->ev_sums) (void)hipEventDestroy(c->ev_sums);
     for (int i = 0; i < c->nev; i++) (void)hipEventDestroy(c->ev[i]);
     free(c->ev);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -321,10 +323,23 @@ int pfc_create(pfc_ctx** out, const pfc_params* p, int Nx, int Ny, int precision
     rc |= dalloc(&c->mreal, c->U * c->nr * rsz);
     rc |= dalloc(&c->R, c->nO * c->nr * rsz);
     rc |= dalloc((void**)&c->f, c->nr * sizeof(double));
-    // energy partials: per block 1 + U sums (f or its local part, each n_u)
-    // and the spectral part of pfc_evaluate_sums
-    rc |= dalloc((void**)&c->partials, (size_t)kEnergyBlocks * 2 * (2 + c->U) * sizeof(double));
+    // energy partials: per block 1 + U sums (f or its local part, each n_u),
+    // for up to max(kEnergyBlocks, Ny) blocks (a step's row pass), then from
+    // poff the spectral part, up to max(kEnergyBlocks, Nx/2 + 1) blocks (a
+    // step's column pass)
+    c->poff = (size_t)std::max(kEnergyBlocks, Ny) * 2 * (1 + c->U);
+    rc |= dalloc((void**)&c->partials,
+                 (c->poff + (size_t)std::max(kEnergyBlocks, Nx / 2 + 1) * 2) * sizeof(double));
     rc |= dalloc((void**)&c->sums, (2 + c->U) * sizeof(double));
+    if (hipHostMalloc((void**)&c->hsums, (2 + c->U) * sizeof(double), hipHostMallocDefault) !=
+        hipSuccess) {
+        c->hsums = nullptr;
+        rc |= PFC_ENOMEM;
+    }
+    if (hipEventCreateWithFlags(&c->ev_sums, hipEventDisableTiming) != hipSuccess) {
+        c->ev_sums = nullptr;
+        rc |= PFC_EDEVICE;
+    }
     rc |= dalloc((void**)&c->nan_flag, sizeof(int));
     rc |= dalloc((void**)&c->dprm, sizeof(DevParams));
     // the k = 0 state and its alternate of the per-field column pass
@@ -594,7 +609,7 @@ int pfc_evaluate_sums(pfc_ctx* c, double* fave, double* nave) {
     hipLaunchKernelGGL((k_reduce_final<1024>), dim3(1), dim3(1024), 0, c->stream, c->partials,
                        nloc, 1 + c->U, c->sums, c->hp.inv_NxNy);
     hipLaunchKernelGGL((k_reduce_final<1024>), dim3(1), dim3(1024), 0, c->stream,
-                       c->partials + (size_t)kEnergyBlocks * 2 * (1 + c->U), kEnergyBlocks, 1,
+                       c->partials + c->poff, kEnergyBlocks, 1,
                        c->sums + 1 + c->U, 1.0);
     HIPCHK(hipGetLastError());
     double h[2 + PFC_MAX_FIELDS];
@@ -604,6 +619,42 @@ int pfc_evaluate_sums(pfc_ctx* c, double* fave, double* nave) {
     *fave = h[0] + h[1 + c->U];
     for (int u = 0; u < c->U; u++) nave[u] = h[1 + u];
     c->f_stale = 1;
+    return PFC_OK;
+}
+
+int pfc_step_evaluate(pfc_ctx* c, int t0, int nsteps, double* fave, double* nave) {
+    if (!c || nsteps < 1 || !fave || !nave) return PFC_EINVAL;
+    if (!c->have_fields || !c->have_box) return PFC_ESTATE;
+    if (c->generic) {
+        const int rc = pfc_evaluate_sums(c, fave, nave);
+        return rc ? rc : pfc_step(c, t0, nsteps);
+    }
+    // step t0 as a refresh step that also writes the evaluate's partial sums
+    c->nh_fresh = 0;
+    int nrow, ncol;
+    if (c->prec == PFC_FP64) {
+        nrow = Launch<double>::row_step_any(c, t0, true, c->partials);
+        ncol = Launch<double>::col_step_any(c, true, c->partials + c->poff);
+    } else {
+        nrow = Launch<float>::row_step_any(c, t0, true, c->partials);
+        ncol = Launch<float>::col_step_any(c, true, c->partials + c->poff);
+    }
+    hipLaunchKernelGGL((k_reduce_final<1024>), dim3(1), dim3(1024), 0, c->stream, c->partials, nrow,
+                       1 + c->U, c->sums, c->hp.inv_NxNy);
+    hipLaunchKernelGGL((k_reduce_final<1024>), dim3(1), dim3(1024), 0, c->stream,
+                       c->partials + c->poff, ncol, 1, c->sums + 1 + c->U, 1.0);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(c->hsums, c->sums, (2 + c->U) * sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipEventRecord(c->ev_sums, c->stream));
+    c->f_stale = 1;
+    if (nsteps > 1) {
+        const int rc = pfc_step(c, t0 + 1, nsteps - 1);
+        if (rc) return rc;
+    }
+    HIPCHK(hipEventSynchronize(c->ev_sums));
+    *fave = c->hsums[0] + c->hsums[1 + c->U];
+    for (int u = 0; u < c->U; u++) nave[u] = c->hsums[1 + u];
     return PFC_OK;
 }
 

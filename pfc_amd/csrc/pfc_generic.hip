@@ -425,7 +425,8 @@ struct Gen {
     template <int U>
     static void sums_u(pfc_ctx* c) {
         launch_energy_sums<T, U>((const T*)c->mreal, (const C*)c->Nh, c->nr, c->nkn, c->Nx, c->Ny,
-                                 c->partials, kEnergyBlocks, c->dprm, c->stream);
+                                 c->partials, c->partials + c->poff, kEnergyBlocks, c->dprm,
+                                 c->stream);
     }
     // the local and spectral energy sums from mreal and Nh
     static void energy_sums(pfc_ctx* c) {

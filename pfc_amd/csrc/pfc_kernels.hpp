@@ -481,6 +481,131 @@ __device__ __forceinline__ void nonlinear(const DevParams* __restrict__ p, T (&m
     }
 }
 
+// ------------------------------------------------------------- reductions
+
+// Compensated accumulation (Neumaier); pairs combine with TwoSum.
+struct KSum {
+    double s, c;
+};
+__device__ __forceinline__ void kadd(KSum& k, double x) {
+    const double t = k.s + x;
+    if (fabs(k.s) >= fabs(x)) k.c += (k.s - t) + x;
+    else k.c += (x - t) + k.s;
+    k.s = t;
+}
+__device__ __forceinline__ KSum kmerge(KSum a, KSum b) {
+    KSum r;
+    r.s = a.s + b.s;
+    const double bb = r.s - a.s;
+    const double e = (a.s - (r.s - bb)) + (b.s - bb);
+    r.c = a.c + b.c + e;
+    return r;
+}
+__device__ __forceinline__ KSum kshfl_down(KSum a, int off) {
+    KSum r;
+    r.s = __shfl_down(a.s, off, 64);
+    r.c = __shfl_down(a.c, off, 64);
+    return r;
+}
+
+// Block reduction of NV compensated sums (wavefront shuffles, then an LDS
+// tree over the block's waves); thread 0 returns the totals.
+template <int NV, int BS>
+__device__ __forceinline__ void block_ksum(KSum (&v)[NV]) {
+    __shared__ KSum sh[BS / 64][NV];
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < NV; i++) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v[i] = kmerge(v[i], kshfl_down(v[i], off));
+        if (lane == 0) sh[wv][i] = v[i];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; i++) {
+            KSum a = sh[0][i];
+            for (int w = 1; w < BS / 64; w++) a = kmerge(a, sh[w][i]);
+            v[i] = a;
+        }
+    }
+}
+
+// Multiplier of one evaluate output at k2 (src/pfc-relax.c:1048-1077):
+// kind 0: P_u(k) (the multimode operator), 1: G(k) (Gaussian),
+// 2: (c2 - k2)^2 (gradient coupling, c2 = mu2 or the reference's nu2 alias).
+__device__ __forceinline__ double eval_mult(const DevParams* __restrict__ p, int kind, int u,
+                                            double c2, double k2) {
+    if (kind == 0) {
+        double P = 1.0;
+        for (int w = 0; w < p->W[u]; w++) {
+            const double pw = p->nu2[u][w] - k2;
+            P *= p->omega[u][w] * pw * pw + p->phi[u][w];
+        }
+        return P;
+    } else if (kind == 1) {
+        return exp(p->sig_a * k2);
+    } else {
+        const double d = c2 - k2;
+        return d * d;
+    }
+}
+
+// The spectral energy terms of one half-spectrum point (Parseval; see
+// k_energy_spec): 1/2 beta_uu P_u |N_u|^2 + (beta_uv (c2 - k2)^2 + eps_uv G^2)
+// Re(N_u conj N_v), counted twice for the columns 0 < kx < Nx/2 that the
+// C2R along x takes as 2 Re(.).
+template <int U, typename T>
+__device__ __forceinline__ double spec_energy(const DevParams* __restrict__ p, int kx, int Nx,
+                                              double k2, const cpx<T> (&N)[U]) {
+    double re[U], im[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        re[u] = (double)N[u].x;
+        im[u] = (double)N[u].y;
+    }
+    const double G = exp_cw(p->sig_a * k2);   // eval_mult's G(k), this file's exp
+    const double G2 = G * G;
+    double e = 0.0;
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        e += 0.5 * p->beta[u][u] * eval_mult(p, 0, u, 0.0, k2) * (re[u] * re[u] + im[u] * im[u]);
+#pragma unroll
+        for (int v = u + 1; v < U; v++) {
+            const double m = p->beta[u][v] * eval_mult(p, 2, u, p->qc2[u][v], k2) + p->epsilon[u][v] * G2;
+            e += m * (re[u] * re[v] + im[u] * im[v]);
+        }
+    }
+    const bool once = (kx == 0) || (2 * kx == Nx);
+    return once ? e : 2.0 * e;
+}
+
+// The local terms of the free energy at one point (src/pfc-relax.c:1093-1117
+// without the filtered fields): lambda n + alpha n^2/2 + gamma n^3/3 + delta
+// n^4/4 per field and the pair products, accumulated into acc[0], each n_u
+// into acc[1 + u].
+template <int U>
+__device__ __forceinline__ void local_energy(const DevParams* __restrict__ p, const double (&nv)[U],
+                                             KSum* acc) {
+    const double _3 = 1.0 / 3.0;
+    double fij = 0.0;
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const double nu = nv[u];
+        const double nunu = nu * nu;
+        fij += p->lambda[u] * nu + 0.5 * p->alpha[u][u] * nunu + _3 * p->gamma[u][u] * nu * nunu +
+               0.25 * p->delta[u] * nunu * nunu;
+#pragma unroll
+        for (int v = u + 1; v < U; v++) {
+            const double nvv = nv[v];
+            fij += p->alpha[u][v] * nu * nvv + 0.5 * p->gamma[u][v] * (nunu * nvv + nu * nvv * nvv);
+        }
+        kadd(acc[1 + u], nu);
+    }
+    kadd(acc[0], fij);
+}
+
 // ================================================================ kernels
 
 // LDS index padding of the step kernels' FP32 transforms (BlockFFT PADC;
@@ -542,11 +667,15 @@ __device__ __forceinline__ void row_tab_gather(cpx<T> (&tv)[NI], GBuf twx, int t
 // stores complete in issue order under one counter, so twiddle loads issued
 // during the transforms would make the C2R of field 0 wait for every field's
 // loads and the R2C of field u + 1 wait until field u's stores have drained.
-template <typename T, int NX, int TPB, int TR, int U, bool REFRESH>
+// With EVAL (a refresh step that also evaluates the free energy of the
+// incoming state, pfc_step_evaluate), the local energy terms of the
+// densities are added to eacc[0..U] as k_row_c2r_eloc adds them.
+template <typename T, int NX, int TPB, int TR, int U, bool REFRESH, bool EVAL = false>
 __device__ __forceinline__ void row_step_core(
     cpx<T> (&m)[U][NX / 2 / TPB], cpx<T>* L, const cpx<T>* ltab, GBuf twx, int t, int y,
     cpx<T>* __restrict__ J, cpx<T>* __restrict__ K, size_t fstride, int sJ,
-    const DevParams* __restrict__ prm, int* __restrict__ nan_step, int tstep) {
+    const DevParams* __restrict__ prm, int* __restrict__ nan_step, int tstep,
+    KSum* eacc = nullptr) {
     constexpr int H = NX / 2;
     constexpr int P = H / TPB;
     constexpr unsigned E = sizeof(cpx<T>);
@@ -583,6 +712,18 @@ __device__ __forceinline__ void row_step_core(
     if (y == 0 && t == 0) {
         const T v0 = m[0][0].x;
         if (v0 != v0) atomicMin(nan_step, tstep);
+    }
+    if constexpr (EVAL) {
+#pragma unroll
+        for (int q = 0; q < P; q++) {
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                double nv[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) nv[u] = (double)(h ? m[u][q].y : m[u][q].x);
+                local_energy<U>(prm, nv, eacc);
+            }
+        }
     }
     if constexpr (REFRESH) {
         // R2C of the density itself, for N = FFT(n)/NxNy (src/pfc-relax.c:1178-1184)
@@ -626,11 +767,12 @@ __device__ __forceinline__ void row_step_core(
 // (16-byte aligned).  The table gather is issued before the field loads and
 // written to LDS after them, so that the write waits for the gather only.
 // Shared by the one-system kernel below and the batch kernels.
-template <typename T, int NX, int TPB, int TR, int U, bool REFRESH>
+template <typename T, int NX, int TPB, int TR, int U, bool REFRESH, bool EVAL = false>
 __device__ __forceinline__ void row_step_body(
     cpx<T>* lds, cpx<T>* ltab, int g, const cpx<T>* __restrict__ I, cpx<T>* __restrict__ J,
     cpx<T>* __restrict__ K, size_t fstride, int sI, int sJ, const cpx<T>* __restrict__ twx_,
-    const DevParams* __restrict__ prm, int* __restrict__ nan_step, int tstep) {
+    const DevParams* __restrict__ prm, int* __restrict__ nan_step, int tstep,
+    KSum* eacc = nullptr) {
     constexpr int H = NX / 2;
     constexpr int P = H / TPB;
     constexpr int NT = TPB * TR;
@@ -662,23 +804,46 @@ __device__ __forceinline__ void row_step_body(
 #pragma unroll
         for (int i = 0; i < NI; i++) ltab[tid + i * NT] = tv[i];
     }
-    row_step_core<T, NX, TPB, TR, U, REFRESH>(m, lds + r, ltab, twx, t, y, J, K, fstride, sJ, prm,
-                                              nan_step, tstep);
+    row_step_core<T, NX, TPB, TR, U, REFRESH, EVAL>(m, lds + r, ltab, twx, t, y, J, K, fstride, sJ,
+                                                    prm, nan_step, tstep, eacc);
 }
 
-template <typename T, int NX, int TPB, int TR, int U, bool REFRESH>
-__global__ __launch_bounds__(TPB* TR) PFC_WAVES((row_waves<T, NX>())) void k_row_step(
+// Block totals of the EVAL variants' compensated sums: nv sums per block at
+// part[block * 2 nv].
+template <int NV, int BS>
+__device__ __forceinline__ void eval_partials(KSum (&acc)[NV], double* __restrict__ part) {
+    block_ksum<NV, BS>(acc);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; i++) {
+            part[(size_t)blockIdx.x * 2 * NV + 2 * i] = acc[i].s;
+            part[(size_t)blockIdx.x * 2 * NV + 2 * i + 1] = acc[i].c;
+        }
+    }
+}
+
+// The EVAL variants (one step per evaluate) run at most two waves per SIMD,
+// so that the energy accumulators do not push the FP32 kernels into spills.
+template <typename T, int NX, int TPB, int TR, int U, bool REFRESH, bool EVAL = false>
+__global__ __launch_bounds__(TPB* TR) PFC_WAVES((EVAL ? 2 : row_waves<T, NX>())) void k_row_step(
     const cpx<T>* __restrict__ I, cpx<T>* __restrict__ J, cpx<T>* __restrict__ K,
     size_t fstride, int sI, int sJ, const cpx<T>* __restrict__ twx_,
-    const DevParams* __restrict__ prm, int* __restrict__ nan_step, int tstep) {
+    const DevParams* __restrict__ prm, int* __restrict__ nan_step, int tstep,
+    double* __restrict__ epart) {
     __shared__ cpx<T> lds[row_lds_elems<T, NX, TPB, TR>()];
     __shared__ __attribute__((aligned(16))) cpx<T> ltab[row_tab_size<T, NX, TPB, TR>()];
 #ifndef PFC_ROW_XCD
 #define PFC_ROW_XCD 1
 #endif
     const int g = PFC_ROW_XCD ? xcd_remap(blockIdx.x, gridDim.x) : (int)blockIdx.x;
-    row_step_body<T, NX, TPB, TR, U, REFRESH>(lds, ltab, g, I, J, K, fstride, sI, sJ, twx_, prm,
-                                              nan_step, tstep);
+    KSum acc[EVAL ? 1 + U : 1];
+    if constexpr (EVAL) {
+#pragma unroll
+        for (int i = 0; i <= U; i++) acc[i] = {0.0, 0.0};
+    }
+    row_step_body<T, NX, TPB, TR, U, REFRESH, EVAL>(lds, ltab, g, I, J, K, fstride, sI, sJ, twx_,
+                                                    prm, nan_step, tstep, acc);
+    if constexpr (EVAL) eval_partials<1 + U, TPB * TR>(acc, epart);
 }
 
 // ------------------------------------------------------- persistent passes
@@ -722,11 +887,12 @@ __device__ __forceinline__ void row_tab_load(cpx<T>* ltab, const cpx<T>* __restr
 // Persistent row pass over ngroups row groups of TR rows; the table written
 // before the loop is complete at the first barrier of c2r_pre, which also
 // separates each group's LDS use from the previous group's.
-template <typename T, int NX, int TPB, int TR, int U, bool REFRESH>
-__global__ __launch_bounds__(TPB* TR) PFC_WAVES((row_waves<T, NX>())) void k_row_step_p(
+template <typename T, int NX, int TPB, int TR, int U, bool REFRESH, bool EVAL = false>
+__global__ __launch_bounds__(TPB* TR) PFC_WAVES((EVAL ? 2 : row_waves<T, NX>())) void k_row_step_p(
     const cpx<T>* __restrict__ I, cpx<T>* __restrict__ J, cpx<T>* __restrict__ K,
     size_t fstride, int sI, int sJ, const cpx<T>* __restrict__ twx_,
-    const DevParams* __restrict__ prm, int* __restrict__ nan_step, int tstep, int ngroups) {
+    const DevParams* __restrict__ prm, int* __restrict__ nan_step, int tstep, int ngroups,
+    double* __restrict__ epart) {
     __shared__ cpx<T> lds[row_lds_elems<T, NX, TPB, TR>()];
     __shared__ __attribute__((aligned(16))) cpx<T> ltab[row_tab_size<T, NX, TPB, TR>()];
     constexpr int H = NX / 2;
@@ -738,6 +904,11 @@ __global__ __launch_bounds__(TPB* TR) PFC_WAVES((row_waves<T, NX>())) void k_row
     const int t = tid / TR;
     const GBuf twx = gbuf(twx_);
     const unsigned istep = lay_row_step(sI, TPB, E);
+    KSum acc[EVAL ? 1 + U : 1];
+    if constexpr (EVAL) {
+#pragma unroll
+        for (int i = 0; i <= U; i++) acc[i] = {0.0, 0.0};
+    }
     for (int it = 0;; it++) {
         const int g = persist_row_group(blockIdx.x, gridDim.x, it, ngroups);
         if (g >= ngroups) break;
@@ -750,9 +921,10 @@ __global__ __launch_bounds__(TPB* TR) PFC_WAVES((row_waves<T, NX>())) void k_row
 #pragma unroll
             for (int q = 0; q < P; q++) m[u][q] = gload<T>(src, ioff, q * istep);
         }
-        row_step_core<T, NX, TPB, TR, U, REFRESH>(m, lds + r, ltab, twx, t, y, J, K, fstride, sJ, prm,
-                                                  nan_step, tstep);
+        row_step_core<T, NX, TPB, TR, U, REFRESH, EVAL>(m, lds + r, ltab, twx, t, y, J, K, fstride,
+                                                        sJ, prm, nan_step, tstep, acc);
     }
+    if constexpr (EVAL) eval_partials<1 + U, TPB * TR>(acc, epart);
 }
 
 // Column addressing modes of the column passes.  mode 0: an ordinary column
@@ -839,12 +1011,17 @@ constexpr int col_tw_entries() {
 // (col_tw_entries, 16-byte aligned) holds the twiddle table, and with LGY,
 // lgy (NY reals) the ky factors (read from global memory otherwise).  Shared
 // by the one-system kernel below and the batch kernels.
-template <typename T, int NY, int TPB, int TC, int U, bool REFRESH, bool LTAB, bool LGY>
+// With EVAL (and REFRESH), the spectral energy terms of the refreshed state
+// are added to eacc[0] before the update, as k_energy_spec adds them.
+template <typename T, int NY, int TPB, int TC, int U, bool REFRESH, bool LTAB, bool LGY,
+          bool EVAL = false>
 __device__ __forceinline__ void col_step_body(
     cpx<T>* lds, cpx<T>* ltw, T* lgy, int cb, const cpx<T>* __restrict__ J,
     const cpx<T>* __restrict__ K, cpx<T>* __restrict__ Nh, cpx<T>* __restrict__ I, size_t fstride,
     size_t nstride, int H, int sI, int sJ, const cpx<T>* __restrict__ twy_,
-    const T* __restrict__ ktab_, const DevParams* __restrict__ prm, double* __restrict__ n0) {
+    const T* __restrict__ ktab_, const DevParams* __restrict__ prm, double* __restrict__ n0,
+    KSum* eacc = nullptr) {
+    static_assert(!EVAL || REFRESH, "the evaluate reads the refreshed state");
     constexpr int P = NY / TPB;
     constexpr unsigned E = sizeof(cpx<T>);
     using FFTF = BlockFFT<T, NY, TPB, -1, 1, 1, step_padc<T>()>;
@@ -961,6 +1138,7 @@ __device__ __forceinline__ void col_step_body(
                     const int j = t + q * TPB;
                     const double ky = ky_of(j, NY, prm->dky);
                     const T k2 = (T)(kx2 + ky * ky);
+                    if constexpr (EVAL) kadd(eacc[0], spec_energy<U, T>(prm, kx, 2 * H, kx2 + ky * ky, Nb[i]));
                     T gy;
                     if constexpr (LGY) gy = lgy[j];
                     else gy = gload_real<T>(ktab, (unsigned)(ktab_gy(H) + j) * sizeof(T));
@@ -1043,12 +1221,12 @@ constexpr bool col_lgy() {
                    col_tw_entries<T, NY, TPB, TC, true>() * sizeof(cpx<T>) + NY * sizeof(T) <= 160 * 1024;
 }
 
-template <typename T, int NY, int TPB, int TC, int U, bool REFRESH>
+template <typename T, int NY, int TPB, int TC, int U, bool REFRESH, bool EVAL = false>
 __global__ __launch_bounds__(TPB* TC) PFC_WAVES(col_waves<T>()) void k_col_step(
     const cpx<T>* __restrict__ J, const cpx<T>* __restrict__ K, cpx<T>* __restrict__ Nh,
     cpx<T>* __restrict__ I, size_t fstride, size_t nstride, int H, int sI, int sJ,
     const cpx<T>* __restrict__ twy_, const T* __restrict__ ktab_,
-    const DevParams* __restrict__ prm, double* __restrict__ n0) {
+    const DevParams* __restrict__ prm, double* __restrict__ n0, double* __restrict__ epart) {
     constexpr bool LTAB = col_ltab<T, NY, TPB, TC>();
     constexpr bool LGY = col_lgy<T, NY, TPB, TC>();
     __shared__ cpx<T> lds[col_lds_len<T, NY, TPB>() * TC];
@@ -1056,16 +1234,18 @@ __global__ __launch_bounds__(TPB* TC) PFC_WAVES(col_waves<T>()) void k_col_step(
     // with J in column groups, neighbouring column blocks on one XCD (they
     // read parts of the same lines)
     const int cb = lay_x(sJ) >= 1 ? xcd_remap_any(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+    KSum acc[1] = {{0.0, 0.0}};
     if constexpr (LGY) {
         __shared__ __attribute__((aligned(16))) T lgy[NY];   // exp(-ky^2/sigma^2)
-        col_step_body<T, NY, TPB, TC, U, REFRESH, LTAB, LGY>(lds, ltw, lgy, cb, J, K, Nh, I,
-                                                             fstride, nstride, H, sI, sJ, twy_,
-                                                             ktab_, prm, n0);
+        col_step_body<T, NY, TPB, TC, U, REFRESH, LTAB, LGY, EVAL>(lds, ltw, lgy, cb, J, K, Nh, I,
+                                                                   fstride, nstride, H, sI, sJ,
+                                                                   twy_, ktab_, prm, n0, acc);
     } else {
-        col_step_body<T, NY, TPB, TC, U, REFRESH, LTAB, LGY>(lds, ltw, nullptr, cb, J, K, Nh,
-                                                             I, fstride, nstride, H, sI, sJ, twy_,
-                                                             ktab_, prm, n0);
+        col_step_body<T, NY, TPB, TC, U, REFRESH, LTAB, LGY, EVAL>(lds, ltw, nullptr, cb, J, K, Nh,
+                                                                   I, fstride, nstride, H, sI, sJ,
+                                                                   twy_, ktab_, prm, n0, acc);
     }
+    if constexpr (EVAL) eval_partials<1, TPB * TC>(acc, epart);
 }
 
 // ------------------------------------------------ per-field column pass
@@ -1410,25 +1590,6 @@ __global__ __launch_bounds__(TPB* TC) void k_col_smooth(const cpx<T>* __restrict
     }
 }
 
-// Multiplier of one evaluate output at k2 (src/pfc-relax.c:1048-1077):
-// kind 0: P_u(k) (the multimode operator), 1: G(k) (Gaussian),
-// 2: (c2 - k2)^2 (gradient coupling, c2 = mu2 or the reference's nu2 alias).
-__device__ __forceinline__ double eval_mult(const DevParams* __restrict__ p, int kind, int u,
-                                            double c2, double k2) {
-    if (kind == 0) {
-        double P = 1.0;
-        for (int w = 0; w < p->W[u]; w++) {
-            const double pw = p->nu2[u][w] - k2;
-            P *= p->omega[u][w] * pw * pw + p->phi[u][w];
-        }
-        return P;
-    } else if (kind == 1) {
-        return exp(p->sig_a * k2);
-    } else {
-        const double d = c2 - k2;
-        return d * d;
-    }
-}
 
 // Evaluate's k-space pass for source field s = blockIdx.y: M = FFT_y(K_s)/NxNy
 // (also stored as the refreshed state Nh_s, src/pfc-relax.c:1124-1128), then
@@ -1501,56 +1662,6 @@ __global__ __launch_bounds__(TPB* TC) void k_col_eval(const cpx<T>* __restrict__
     }
 }
 
-// ------------------------------------------------------------- reductions
-
-// Compensated accumulation (Neumaier); pairs combine with TwoSum.
-struct KSum {
-    double s, c;
-};
-__device__ __forceinline__ void kadd(KSum& k, double x) {
-    const double t = k.s + x;
-    if (fabs(k.s) >= fabs(x)) k.c += (k.s - t) + x;
-    else k.c += (x - t) + k.s;
-    k.s = t;
-}
-__device__ __forceinline__ KSum kmerge(KSum a, KSum b) {
-    KSum r;
-    r.s = a.s + b.s;
-    const double bb = r.s - a.s;
-    const double e = (a.s - (r.s - bb)) + (b.s - bb);
-    r.c = a.c + b.c + e;
-    return r;
-}
-__device__ __forceinline__ KSum kshfl_down(KSum a, int off) {
-    KSum r;
-    r.s = __shfl_down(a.s, off, 64);
-    r.c = __shfl_down(a.c, off, 64);
-    return r;
-}
-
-// Block reduction of NV compensated sums (wavefront shuffles, then an LDS
-// tree over the block's waves); thread 0 returns the totals.
-template <int NV, int BS>
-__device__ __forceinline__ void block_ksum(KSum (&v)[NV]) {
-    __shared__ KSum sh[BS / 64][NV];
-    const int lane = threadIdx.x & 63;
-    const int wv = threadIdx.x >> 6;
-#pragma unroll
-    for (int i = 0; i < NV; i++) {
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) v[i] = kmerge(v[i], kshfl_down(v[i], off));
-        if (lane == 0) sh[wv][i] = v[i];
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int i = 0; i < NV; i++) {
-            KSum a = sh[0][i];
-            for (int w = 1; w < BS / 64; w++) a = kmerge(a, sh[w][i]);
-            v[i] = a;
-        }
-    }
-}
 
 // Pointwise free-energy density (src/pfc-relax.c:1093-1117) with per-block
 // compensated partial sums of f and of every n_u.  R holds the real-space
@@ -1619,26 +1730,11 @@ __global__ __launch_bounds__(BS) void k_energy_local(const T* __restrict__ n, si
     KSum acc[1 + U];
 #pragma unroll
     for (int i = 0; i <= U; i++) acc[i] = {0.0, 0.0};
-    const double _3 = 1.0 / 3.0;
     for (size_t ij = (size_t)blockIdx.x * BS + threadIdx.x; ij < nr; ij += (size_t)gridDim.x * BS) {
         double nv[U];
 #pragma unroll
         for (int u = 0; u < U; u++) nv[u] = (double)n[u * nr + ij];
-        double fij = 0.0;
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            const double nu = nv[u];
-            const double nunu = nu * nu;
-            fij += p->lambda[u] * nu + 0.5 * p->alpha[u][u] * nunu + _3 * p->gamma[u][u] * nu * nunu +
-                   0.25 * p->delta[u] * nunu * nunu;
-#pragma unroll
-            for (int v = u + 1; v < U; v++) {
-                const double nvv = nv[v];
-                fij += p->alpha[u][v] * nu * nvv + 0.5 * p->gamma[u][v] * (nunu * nvv + nu * nvv * nvv);
-            }
-            kadd(acc[1 + u], nu);
-        }
-        kadd(acc[0], fij);
+        local_energy<U>(p, nv, acc);
     }
     block_ksum<1 + U, BS>(acc);
     if (threadIdx.x == 0) {
@@ -1688,7 +1784,6 @@ __global__ __launch_bounds__(TPB* TR) void k_row_c2r_eloc(
     const int t = tid / TR;
     cpx<T>* L = lds + r;
     const unsigned istep = lay_row_step(sI, TPB, E), kstep = lay_row_step(sK, TPB, E);
-    const double _3 = 1.0 / 3.0;
     KSum acc[1 + U];
 #pragma unroll
     for (int i = 0; i <= U; i++) acc[i] = {0.0, 0.0};
@@ -1727,22 +1822,7 @@ __global__ __launch_bounds__(TPB* TR) void k_row_c2r_eloc(
                 double nv[U];
 #pragma unroll
                 for (int u = 0; u < U; u++) nv[u] = (double)(h ? m[u][q].y : m[u][q].x);
-                double fij = 0.0;
-#pragma unroll
-                for (int u = 0; u < U; u++) {
-                    const double nu = nv[u];
-                    const double nunu = nu * nu;
-                    fij += p->lambda[u] * nu + 0.5 * p->alpha[u][u] * nunu +
-                           _3 * p->gamma[u][u] * nu * nunu + 0.25 * p->delta[u] * nunu * nunu;
-#pragma unroll
-                    for (int v = u + 1; v < U; v++) {
-                        const double nvv = nv[v];
-                        fij += p->alpha[u][v] * nu * nvv +
-                               0.5 * p->gamma[u][v] * (nunu * nvv + nu * nvv * nvv);
-                    }
-                    kadd(acc[1 + u], nu);
-                }
-                kadd(acc[0], fij);
+                local_energy<U>(p, nv, acc);
             }
         }
     }
@@ -1767,27 +1847,10 @@ __global__ __launch_bounds__(BS) void k_energy_spec(const cpx<T>* __restrict__ N
         const double kxd = kx * p->dkx;
         const double ky = ky_of(j, Ny, p->dky);
         const double k2 = kxd * kxd + ky * ky;
-        double re[U], im[U];
+        cpx<T> z[U];
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-            const cpx<T> z = Nh[u * nkn + idx];
-            re[u] = (double)z.x;
-            im[u] = (double)z.y;
-        }
-        const double G = eval_mult(p, 1, 0, 0.0, k2);
-        const double G2 = G * G;
-        double e = 0.0;
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            e += 0.5 * p->beta[u][u] * eval_mult(p, 0, u, 0.0, k2) * (re[u] * re[u] + im[u] * im[u]);
-#pragma unroll
-            for (int v = u + 1; v < U; v++) {
-                const double m = p->beta[u][v] * eval_mult(p, 2, u, p->qc2[u][v], k2) + p->epsilon[u][v] * G2;
-                e += m * (re[u] * re[v] + im[u] * im[v]);
-            }
-        }
-        const bool once = (kx == 0) || (2 * kx == Nx);
-        kadd(acc[0], once ? e : 2.0 * e);
+        for (int u = 0; u < U; u++) z[u] = Nh[u * nkn + idx];
+        kadd(acc[0], spec_energy<U, T>(p, kx, Nx, k2, z));
     }
     block_ksum<1, BS>(acc);
     if (threadIdx.x == 0) {
@@ -1800,10 +1863,11 @@ __global__ __launch_bounds__(BS) void k_energy_spec(const cpx<T>* __restrict__ N
 // spectral ones after them.
 template <typename T, int U>
 void launch_energy_sums(const T* n, const cpx<T>* Nh, size_t nr, size_t nkn, int Nx, int Ny,
-                        double* partials, int nblk, const DevParams* p, hipStream_t s) {
+                        double* partials, double* spec_partials, int nblk, const DevParams* p,
+                        hipStream_t s) {
     hipLaunchKernelGGL((k_energy_local<T, U, 256>), dim3(nblk), dim3(256), 0, s, n, nr, partials, p);
     hipLaunchKernelGGL((k_energy_spec<T, U, 256>), dim3(nblk), dim3(256), 0, s, Nh, nkn, Nx, Ny,
-                       partials + (size_t)nblk * 2 * (1 + U), p);
+                       spec_partials, p);
 }
 
 // Second stage: one block folds nblk partial pairs per quantity.

@@ -134,7 +134,12 @@ struct pfc_ctx {
     double* n0;   // mixed form: the k = 0 state of each field in FP64 (re, im)
     void* ktab;   // Gaussian factors exp(-k^2/sigma^2) per kx and per ky (ktab_gy), set by pfc_set_box
     void* hktab;  // host staging of ktab
-    double *f, *partials, *sums;
+    // partials: the energy sums' per-block partial pairs, local ones (1 + U
+    // per block) from 0 and spectral ones (1 per block) from poff; sums:
+    // their totals; hsums: pinned host copy for pfc_step_evaluate
+    double *f, *partials, *sums, *hsums;
+    size_t poff;
+    hipEvent_t ev_sums;
     int* nan_flag;
     hipStream_t stream;
     int ncu;       // compute units of the device
@@ -181,8 +186,10 @@ void gen_smooth(pfc_ctx* c, const void* in, void* out, double sigma);
 // them and pfc_launch_f64.hip / pfc_launch_f32.hip instantiate them.
 template <typename T>
 struct Launch {
-    static void row_step_any(pfc_ctx* c, int t, bool ref);
-    static void col_step_any(pfc_ctx* c, bool ref);
+    // a step's row and column passes; epart (refresh steps only): also the
+    // free energy's partial sums of the incoming state (returns the blocks)
+    static int row_step_any(pfc_ctx* c, int t, bool ref, double* epart = nullptr);
+    static int col_step_any(pfc_ctx* c, bool ref, double* epart = nullptr);
     // per-field column pass of a non-refresh step: state read from src,
     // written to dst (both U * nkn elements); false if not built for Ny
     static bool col_step_split_any(pfc_ctx* c, const void* src, void* dst);

@@ -27,98 +27,109 @@ template <typename T>
 struct LaunchImpl {
     using C = cpx<T>;
 
-    template <int NX, int U, bool REF>
-    static void row_step(pfc_ctx* c, int t) {
+    // The step's passes; with epart (a refresh step only) the EVAL variants,
+    // which also write the free energy's partial sums of the incoming state
+    // there.  Return the blocks launched (the partial sums written).
+    template <int NX, int U, bool REF, bool EV>
+    static int row_step(pfc_ctx* c, int t, double* epart) {
         if constexpr (size_built(NX)) {
             constexpr RowCfg g = row_cfg<T>(NX);
             const int ng = c->Ny / g.tr;
-            bool persistent = false;
             if constexpr (sizeof(T) == 8) {
                 if (c->persist) {
-                    auto k = k_row_step_p<T, NX, g.tpb, g.tr, U, REF>;
+                    auto k = k_row_step_p<T, NX, g.tpb, g.tr, U, REF, EV>;
                     static const int occ = occupancy((const void*)k, g.tpb * g.tr);
-                    hipLaunchKernelGGL(k, dim3(persist_grid(c, occ, ng)), dim3(g.tpb * g.tr), 0,
-                                       c->stream, (const C*)c->I, (C*)c->J, (C*)c->K, c->nk, c->sI,
-                                       c->sJ, (const C*)c->twx, c->dprm, c->nan_flag, t, ng);
-                    persistent = true;
+                    const int nb = persist_grid(c, occ, ng);
+                    hipLaunchKernelGGL(k, dim3(nb), dim3(g.tpb * g.tr), 0, c->stream, (const C*)c->I,
+                                       (C*)c->J, (C*)c->K, c->nk, c->sI, c->sJ, (const C*)c->twx,
+                                       c->dprm, c->nan_flag, t, ng, epart);
+                    return nb;
                 }
             }
-            if (!persistent) {
-                hipLaunchKernelGGL((k_row_step<T, NX, g.tpb, g.tr, U, REF>), dim3(ng),
-                                   dim3(g.tpb * g.tr), 0, c->stream, (const C*)c->I, (C*)c->J, (C*)c->K,
-                                   c->nk, c->sI, c->sJ, (const C*)c->twx, c->dprm, c->nan_flag, t);
-            }
+            hipLaunchKernelGGL((k_row_step<T, NX, g.tpb, g.tr, U, REF, EV>), dim3(ng),
+                               dim3(g.tpb * g.tr), 0, c->stream, (const C*)c->I, (C*)c->J, (C*)c->K,
+                               c->nk, c->sI, c->sJ, (const C*)c->twx, c->dprm, c->nan_flag, t, epart);
+            return ng;
         }
+        return 0;
     }
     template <int NX, int U>
-    static void row_step_u(pfc_ctx* c, int t, bool ref) {
-        if (ref) row_step<NX, U, true>(c, t);
-        else row_step<NX, U, false>(c, t);
+    static int row_step_u(pfc_ctx* c, int t, bool ref, double* epart) {
+        if (epart) return row_step<NX, U, true, true>(c, t, epart);
+        if (ref) return row_step<NX, U, true, false>(c, t, nullptr);
+        return row_step<NX, U, false, false>(c, t, nullptr);
     }
     template <int NX>
-    static void row_step_n(pfc_ctx* c, int t, bool ref) {
+    static int row_step_n(pfc_ctx* c, int t, bool ref, double* epart) {
         switch (c->U) {
-            case 1: row_step_u<NX, 1>(c, t, ref); break;
-            case 2: row_step_u<NX, 2>(c, t, ref); break;
-            case 3: row_step_u<NX, 3>(c, t, ref); break;
-            case 4: row_step_u<NX, 4>(c, t, ref); break;
+            case 1: return row_step_u<NX, 1>(c, t, ref, epart);
+            case 2: return row_step_u<NX, 2>(c, t, ref, epart);
+            case 3: return row_step_u<NX, 3>(c, t, ref, epart);
+            case 4: return row_step_u<NX, 4>(c, t, ref, epart);
         }
+        return 0;
     }
-    static void row_step_any(pfc_ctx* c, int t, bool ref) {
+    static int row_step_any(pfc_ctx* c, int t, bool ref, double* epart) {
         switch (c->Nx) {
-            case 64: row_step_n<64>(c, t, ref); break;
-            case 128: row_step_n<128>(c, t, ref); break;
-            case 256: row_step_n<256>(c, t, ref); break;
-            case 384: row_step_n<384>(c, t, ref); break;
-            case 512: row_step_n<512>(c, t, ref); break;
-            case 768: row_step_n<768>(c, t, ref); break;
-            case 1024: row_step_n<1024>(c, t, ref); break;
-            case 1536: row_step_n<1536>(c, t, ref); break;
-            case 2048: row_step_n<2048>(c, t, ref); break;
-            case 4096: row_step_n<4096>(c, t, ref); break;
-            case 8192: row_step_n<8192>(c, t, ref); break;
+            case 64: return row_step_n<64>(c, t, ref, epart);
+            case 128: return row_step_n<128>(c, t, ref, epart);
+            case 256: return row_step_n<256>(c, t, ref, epart);
+            case 384: return row_step_n<384>(c, t, ref, epart);
+            case 512: return row_step_n<512>(c, t, ref, epart);
+            case 768: return row_step_n<768>(c, t, ref, epart);
+            case 1024: return row_step_n<1024>(c, t, ref, epart);
+            case 1536: return row_step_n<1536>(c, t, ref, epart);
+            case 2048: return row_step_n<2048>(c, t, ref, epart);
+            case 4096: return row_step_n<4096>(c, t, ref, epart);
+            case 8192: return row_step_n<8192>(c, t, ref, epart);
         }
+        return 0;
     }
 
-    template <int NY, int U, bool REF>
-    static void col_step(pfc_ctx* c) {
+    template <int NY, int U, bool REF, bool EV>
+    static int col_step(pfc_ctx* c, double* epart) {
         if constexpr (size_built(NY)) {
             constexpr ColCfg g = col_step_cfg(NY);
             const int H = c->Nx / 2;
-            hipLaunchKernelGGL((k_col_step<T, NY, g.tpb, g.tc, U, REF>), dim3(H / g.tc + 1),
+            hipLaunchKernelGGL((k_col_step<T, NY, g.tpb, g.tc, U, REF, EV>), dim3(H / g.tc + 1),
                                dim3(g.tpb * g.tc), 0, c->stream, (const C*)c->J, (const C*)c->K,
                                (C*)c->Nh, (C*)c->I, c->nk, c->nkn, H, c->sI, c->sJ, (const C*)c->twy,
-                               (const T*)c->ktab, c->dprm, c->n0);
+                               (const T*)c->ktab, c->dprm, c->n0, epart);
+            return H / g.tc + 1;
         }
+        return 0;
     }
     template <int NY, int U>
-    static void col_step_u(pfc_ctx* c, bool ref) {
-        if (ref) col_step<NY, U, true>(c);
-        else col_step<NY, U, false>(c);
+    static int col_step_u(pfc_ctx* c, bool ref, double* epart) {
+        if (epart) return col_step<NY, U, true, true>(c, epart);
+        if (ref) return col_step<NY, U, true, false>(c, nullptr);
+        return col_step<NY, U, false, false>(c, nullptr);
     }
     template <int NY>
-    static void col_step_n(pfc_ctx* c, bool ref) {
+    static int col_step_n(pfc_ctx* c, bool ref, double* epart) {
         switch (c->U) {
-            case 1: col_step_u<NY, 1>(c, ref); break;
-            case 2: col_step_u<NY, 2>(c, ref); break;
-            case 3: col_step_u<NY, 3>(c, ref); break;
-            case 4: col_step_u<NY, 4>(c, ref); break;
+            case 1: return col_step_u<NY, 1>(c, ref, epart);
+            case 2: return col_step_u<NY, 2>(c, ref, epart);
+            case 3: return col_step_u<NY, 3>(c, ref, epart);
+            case 4: return col_step_u<NY, 4>(c, ref, epart);
         }
+        return 0;
     }
-    static void col_step_any(pfc_ctx* c, bool ref) {
+    static int col_step_any(pfc_ctx* c, bool ref, double* epart) {
         switch (c->Ny) {
-            case 64: col_step_n<64>(c, ref); break;
-            case 128: col_step_n<128>(c, ref); break;
-            case 256: col_step_n<256>(c, ref); break;
-            case 384: col_step_n<384>(c, ref); break;
-            case 512: col_step_n<512>(c, ref); break;
-            case 768: col_step_n<768>(c, ref); break;
-            case 1024: col_step_n<1024>(c, ref); break;
-            case 1536: col_step_n<1536>(c, ref); break;
-            case 2048: col_step_n<2048>(c, ref); break;
-            case 4096: col_step_n<4096>(c, ref); break;
-            case 8192: col_step_n<8192>(c, ref); break;
+            case 64: return col_step_n<64>(c, ref, epart);
+            case 128: return col_step_n<128>(c, ref, epart);
+            case 256: return col_step_n<256>(c, ref, epart);
+            case 384: return col_step_n<384>(c, ref, epart);
+            case 512: return col_step_n<512>(c, ref, epart);
+            case 768: return col_step_n<768>(c, ref, epart);
+            case 1024: return col_step_n<1024>(c, ref, epart);
+            case 1536: return col_step_n<1536>(c, ref, epart);
+            case 2048: return col_step_n<2048>(c, ref, epart);
+            case 4096: return col_step_n<4096>(c, ref, epart);
+            case 8192: return col_step_n<8192>(c, ref, epart);
         }
+        return 0;
     }
 
     // per-field column pass (2^k columns); the mixed form's FP64 k = 0
@@ -348,8 +359,7 @@ struct LaunchImpl {
     template <int U>
     static void energy_spec(pfc_ctx* c) {
         hipLaunchKernelGGL((k_energy_spec<T, U, 256>), dim3(kEnergyBlocks), dim3(256), 0, c->stream,
-                           (const C*)c->Nh, c->nkn, c->Nx, c->Ny,
-                           c->partials + (size_t)kEnergyBlocks * 2 * (1 + U), c->dprm);
+                           (const C*)c->Nh, c->nkn, c->Nx, c->Ny, c->partials + c->poff, c->dprm);
     }
     static void energy_spec_any(pfc_ctx* c) {
         switch (c->U) {
@@ -362,7 +372,8 @@ struct LaunchImpl {
     template <int U>
     static void energy_sums(pfc_ctx* c) {
         launch_energy_sums<T, U>((const T*)c->mreal, (const C*)c->Nh, c->nr, c->nkn, c->Nx, c->Ny,
-                                 c->partials, kEnergyBlocks, c->dprm, c->stream);
+                                 c->partials, c->partials + c->poff, kEnergyBlocks, c->dprm,
+                                 c->stream);
     }
     static void energy_sums_any(pfc_ctx* c) {
         switch (c->U) {
@@ -375,9 +386,13 @@ struct LaunchImpl {
 };
 
 template <typename T>
-void Launch<T>::row_step_any(pfc_ctx* c, int t, bool ref) { LaunchImpl<T>::row_step_any(c, t, ref); }
+int Launch<T>::row_step_any(pfc_ctx* c, int t, bool ref, double* epart) {
+    return LaunchImpl<T>::row_step_any(c, t, ref, epart);
+}
 template <typename T>
-void Launch<T>::col_step_any(pfc_ctx* c, bool ref) { LaunchImpl<T>::col_step_any(c, ref); }
+int Launch<T>::col_step_any(pfc_ctx* c, bool ref, double* epart) {
+    return LaunchImpl<T>::col_step_any(c, ref, epart);
+}
 template <typename T>
 bool Launch<T>::col_step_split_any(pfc_ctx* c, const void* src, void* dst) {
     return LaunchImpl<T>::col_step_split_any(c, src, dst);

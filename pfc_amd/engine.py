@@ -25,7 +25,7 @@ SYMBOLS = (
     "pfc_get_fields", "pfc_get_energy", "pfc_nan_step", "pfc_timing", "pfc_timing_read",
     "pfc_destroy", "pfc_strerror", "pfc_smooth", "pfc_fields",
     "pfc_batch_create", "pfc_batch_step", "pfc_batch_shape", "pfc_batch_destroy",
-    "pfc_smooth_noalloc", "pfc_perturb",
+    "pfc_smooth_noalloc", "pfc_perturb", "pfc_step_evaluate",
 )
 
 BATCH_AUTOTUNE = 1
@@ -57,6 +57,7 @@ def load():
     L.pfc_step.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     L.pfc_evaluate.argtypes = [vp, dp, dp]
     L.pfc_evaluate_sums.argtypes = [vp, dp, dp]
+    L.pfc_step_evaluate.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, dp]
     L.pfc_get_fields.argtypes = [vp, dp]
     L.pfc_get_energy.argtypes = [vp, dp]
     L.pfc_nan_step.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
@@ -156,6 +157,16 @@ class Engine:
         nave = np.zeros(self.U)
         fn = "pfc_evaluate" if density else "pfc_evaluate_sums"
         self._check(getattr(load(), fn)(self._h, ctypes.byref(fave), _dp(nave)), fn)
+        return fave.value, nave
+
+    def step_evaluate(self, t0, n):
+        """(fave, nave) of the state at t0 as evaluate(density=False) gives
+        them, then n >= 1 steps from t0 (pfc_step_evaluate: step t0's passes
+        take the sums)."""
+        fave = ctypes.c_double()
+        nave = np.zeros(self.U)
+        self._check(load().pfc_step_evaluate(self._h, t0, n, ctypes.byref(fave), _dp(nave)),
+                    "pfc_step_evaluate")
         return fave.value, nave
 
     def perturb(self, rel, seed):

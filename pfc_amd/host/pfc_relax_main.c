@@ -170,16 +170,8 @@ static void relax(run_state* s) {
     int t = 0;
     while (t <= c->tend) {
         const int print = c->Tprint > 0 && t % c->Tprint == 0;
+        const int write = nw < c->ntwrite && t == c->twrite[nw];
         if (c->Topt > 0 && t > 0 && t % c->Topt == 0) optimize(s, &d, !print && density_due(c, t, nw));
-        if (print) {
-            const double fave = evaluate(s, density_due(c, t, nw));
-            pfc_print_output(c, t, (int)(time(NULL) - s->t0), c->dx, c->dy, fave, s->nave);
-        }
-        if (nw < c->ntwrite && t == c->twrite[nw]) {
-            write_outputs(s, t);
-            nw++;
-        }
-        if (t >= c->tend) break;
         int next = c->tend;
         if (c->Tprint > 0) {
             const int k = (t / c->Tprint + 1) * c->Tprint;
@@ -189,9 +181,31 @@ static void relax(run_state* s) {
             const int k = (t / c->Topt + 1) * c->Topt;
             if (k < next) next = k;
         }
-        if (nw < c->ntwrite && c->twrite[nw] > t && c->twrite[nw] < next) next = c->twrite[nw];
-        int rc = pfc_step(s->ctx, t, next - t);
-        if (rc) gpu_fail(rc, "pfc_step");
+        {
+            const int w = nw + write;
+            if (w < c->ntwrite && c->twrite[w] > t && c->twrite[w] < next) next = c->twrite[w];
+        }
+        int rc;
+        if (print && !write && t < c->tend && !density_due(c, t, nw)) {
+            /* a print with nothing written at t: the evaluate rides on the
+             * first of the steps that follow it (pfc_step_evaluate) */
+            double fave = 0.0;
+            rc = pfc_step_evaluate(s->ctx, t, next - t, &fave, s->nave);
+            if (rc) gpu_fail(rc, "pfc_step_evaluate");
+            pfc_print_output(c, t, (int)(time(NULL) - s->t0), c->dx, c->dy, fave, s->nave);
+        } else {
+            if (print) {
+                const double fave = evaluate(s, density_due(c, t, nw));
+                pfc_print_output(c, t, (int)(time(NULL) - s->t0), c->dx, c->dy, fave, s->nave);
+            }
+            if (write) {
+                write_outputs(s, t);
+                nw++;
+            }
+            if (t >= c->tend) break;
+            rc = pfc_step(s->ctx, t, next - t);
+            if (rc) gpu_fail(rc, "pfc_step");
+        }
         int bad = -1;
         rc = pfc_nan_step(s->ctx, &bad);
         if (rc) gpu_fail(rc, "pfc_nan_step");
