@@ -109,14 +109,15 @@ int pfc_evaluate(pfc_ctx* ctx, double* fave, double* nave);
 int pfc_evaluate_sums(pfc_ctx* ctx, double* fave, double* nave);
 
 /* pfc_evaluate_sums followed by pfc_step(ctx, t0, nsteps), nsteps >= 1, as
- * relax() runs a print and the steps after it when no write and no box
- * search falls on t0 (:1299-1310): step t0 runs as a refresh step (the
- * refresh an evaluate performs, :1126-1128) whose row and column passes also
- * take the evaluate's sums of the incoming state, so the evaluate costs no
- * pass of its own.  fave and nave are those of the state at t0; the call
- * returns once they are known, with the later steps possibly still running
- * on the device.  Afterwards pfc_get_energy fails with PFC_ESTATE until the
- * next pfc_evaluate.
+ * relax() runs a print and the steps after it when no write falls on t0
+ * (:1299-1310).  Where step t0 refreshes the state (t0 % 100 == 0,
+ * :1178-1184), its row and column passes also take the evaluate's sums of
+ * the refreshed state, which is what the evaluate's own refresh (:1126-1128)
+ * computes, so the evaluate costs no pass of its own; elsewhere the two run
+ * one after the other.  fave and nave are those of the state at t0; the
+ * call returns once they are known, with the later steps possibly still
+ * running on the device.  Afterwards pfc_get_energy fails with PFC_ESTATE
+ * until the next pfc_evaluate.
  * Replaces evaluate() (:987-1130) and step() (:1133-1247) of a print step. */
 int pfc_step_evaluate(pfc_ctx* ctx, int t0, int nsteps, double* fave, double* nave);
 

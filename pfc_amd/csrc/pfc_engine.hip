@@ -629,8 +629,19 @@ int pfc_step_evaluate(pfc_ctx* c, int t0, int nsteps, double* fave, double* nave
         const int rc = pfc_evaluate_sums(c, fave, nave);
         return rc ? rc : pfc_step(c, t0, nsteps);
     }
-    // step t0 as a refresh step that also writes the evaluate's partial sums
-    c->nh_fresh = 0;
+    // A refresh step t0 (t0 % 100 == 0, no evaluate since the last step)
+    // runs with the passes that also write the evaluate's partial sums of the
+    // refreshed state: the local terms from the densities the row pass
+    // computes, the spectral ones from the state the column pass refreshes
+    // (FP64 at 4096^2 with three fields: 14.93 against 15.06 ms for a print
+    // and 20 steps).  Otherwise, and in the mixed form (whose FP32 row pass
+    // with the sums' registers runs one block per CU instead of two, and
+    // which gained nothing: 7.16 against 7.15 ms), the evaluate runs on its
+    // own, before the steps.
+    if (t0 % 100 != 0 || c->nh_fresh || c->prec != PFC_FP64) {
+        const int rc = pfc_evaluate_sums(c, fave, nave);
+        return rc ? rc : pfc_step(c, t0, nsteps);
+    }
     int nrow, ncol;
     if (c->prec == PFC_FP64) {
         nrow = Launch<double>::row_step_any(c, t0, true, c->partials);

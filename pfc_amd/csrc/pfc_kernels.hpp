@@ -1011,8 +1011,8 @@ constexpr int col_tw_entries() {
 // (col_tw_entries, 16-byte aligned) holds the twiddle table, and with LGY,
 // lgy (NY reals) the ky factors (read from global memory otherwise).  Shared
 // by the one-system kernel below and the batch kernels.
-// With EVAL (and REFRESH), the spectral energy terms of the refreshed state
-// are added to eacc[0] before the update, as k_energy_spec adds them.
+// With EVAL (refresh steps only) the spectral energy terms of the refreshed
+// state are added to eacc[0] before the update, as k_energy_spec adds them.
 template <typename T, int NY, int TPB, int TC, int U, bool REFRESH, bool LTAB, bool LGY,
           bool EVAL = false>
 __device__ __forceinline__ void col_step_body(
