@@ -55,7 +55,10 @@ struct LaunchImpl {
     }
     template <int NX, int U>
     static int row_step_u(pfc_ctx* c, int t, bool ref, double* epart) {
-        if (epart) return row_step<NX, U, true, true>(c, t, epart);
+        // (the FP64 form only: pfc_step_evaluate takes it for FP64 alone)
+        if constexpr (sizeof(T) == 8) {
+            if (epart) return row_step<NX, U, true, true>(c, t, epart);
+        }
         if (ref) return row_step<NX, U, true, false>(c, t, nullptr);
         return row_step<NX, U, false, false>(c, t, nullptr);
     }
@@ -101,7 +104,9 @@ struct LaunchImpl {
     }
     template <int NY, int U>
     static int col_step_u(pfc_ctx* c, bool ref, double* epart) {
-        if (epart) return col_step<NY, U, true, true>(c, epart);
+        if constexpr (sizeof(T) == 8) {
+            if (epart) return col_step<NY, U, true, true>(c, epart);
+        }
         if (ref) return col_step<NY, U, true, false>(c, nullptr);
         return col_step<NY, U, false, false>(c, nullptr);
     }
