@@ -495,9 +495,11 @@ def main(argv=None):
         dom = "column" if r["col_ms"] >= r["row_ms"] else "row"
         ach = r["col_gbs"] if dom == "column" else r["row_gbs"]
         # the engine's choice of column pass for plain steps (pfc_create):
-        # per field for three fields at Ny >= 4096 (FP64) or 8192 (mixed)
+        # per field for three fields at Ny <= 1024 and Ny >= 4096 (FP64) or
+        # 8192 (mixed)
         split = os.environ.get("PFC_COLSPLIT", "") != "0" and (
-            os.environ.get("PFC_COLSPLIT", "") == "1" or Ny >= (4096 if pname == "fp64" else 8192))
+            os.environ.get("PFC_COLSPLIT", "") == "1" or Ny <= 1024
+            or Ny >= (4096 if pname == "fp64" else 8192))
         col_kernel = "k_col_step_f (column pass per field, " if split else "k_col_step (column pass, "
         return {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": ach / PEAK_HBM_GBS,

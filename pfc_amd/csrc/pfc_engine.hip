@@ -258,15 +258,17 @@ int pfc_create(pfc_ctx** out, const pfc_params* p, int Nx, int Ny, int precision
     {
         const char* e = getenv("PFC_PERSIST");
         c->persist = !(e && e[0] == '0');
-        // The per-field column pass where it measured faster (4096^2 and
-        // 8192^2 with three fields: FP64 column pass 399 -> 363 us and
+        // The per-field column pass where it measured faster, with three
+        // fields: 4096^2 and 8192^2 (FP64 column pass 399 -> 363 us and
         // 2.40 -> 1.71 ms, FP32 at 8192 831 -> 750 us; FP32 columns of 4096,
-        // whose in-place pass already runs two blocks per CU, 162 -> 180 us);
+        // whose in-place pass already runs two blocks per CU, 162 -> 180 us)
+        // and every grid up to 1024^2 (whole steps 13-25 % faster in both
+        // forms: U times the blocks; at 2048^2 1-7 % slower);
         // PFC_COLSPLIT=1 takes it wherever it is built, =0 nowhere.
         e = getenv("PFC_COLSPLIT");
         const bool built = !generic && (precision == PFC_FP64 ? Launch<double>::col_split_built(Ny)
                                                               : Launch<float>::col_split_built(Ny));
-        const bool pays = p->U >= 2 && Ny >= (precision == PFC_FP64 ? 4096 : 8192);
+        const bool pays = p->U >= 2 && (Ny <= 1024 || Ny >= (precision == PFC_FP64 ? 4096 : 8192));
         c->colsplit = built && (e && e[0] ? e[0] == '1' : pays);
     }
     c->nO = 2 * c->U + c->U * (c->U - 1) / 2;
