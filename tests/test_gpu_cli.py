@@ -112,7 +112,7 @@ def test_graphene_run_matches_oracle(gpu, tmp_path, c, topt):
     print(f"c={c} topt={topt}: box {box:.2e} fave {fav:.2e} nave {nav:.2e} density {errs} "
           f"energy density {ferrs}")
     if topt == 0:
-        # no box search: the north star's bars (energy 1e-12, density 1e-10)
+        # no box search: energy to 1e-12, density to 1e-10 relative
         # (the box is printed at 14 decimals: equal up to that rounding)
         assert box < 1e-13 and fav < 1e-12 and nav < 1e-14
         assert max(errs) < 1e-10, errs

@@ -110,7 +110,7 @@ def c4():
 def test_heterostructure_8192(gpu, c4):
     """8192^2, three fields, 128 seeded domains, dt = 0.25: FP64 and mixed
     against the oracle for 2 steps, then mixed against FP64 over 100 steps
-    (energy per area within 1e-6, the north star's bound).  The hetero.in
+    (energy per area within 1e-6).  The hetero.in
     parameters diverge at dt = 0.5 (the CPU oracle reaches NaN within 20
     steps at 256^2), so the demo's dt = 0.25 is used."""
     N, p, f, ref, fo = c4["N"], c4["p"], c4["f"], c4["ref"], c4["fo"]

@@ -5,7 +5,8 @@ U, :123-159).  Transforms: mixed radix 8/4/2/3/5/7/11/13 or Bluestein.
 
 Checked against numpy's FP64 FFT (Gaussian smoothing, smooth.c:73-119) and
 against the CPU oracle (step :1133-1247, evaluate :987-1130, relax
-:1292-1312): FP64 to the north star's bars, mixed to FP32-level bounds."""
+:1292-1312): FP64 to 1e-12 in the energy and 1e-10 in the densities, mixed to
+FP32-level bounds."""
 import math
 import os
 import subprocess
