@@ -320,7 +320,7 @@ int pfc_create(pfc_ctx** out, const pfc_params* p, int Nx, int Ny, int precision
     rc |= dalloc(&c->J, spec);
     rc |= dalloc(&c->K, spec);
     rc |= dalloc(&c->Nh, c->U * c->nkn * c->esz);
-    if (c->colsplit) rc |= dalloc(&c->Nh2, c->U * c->nkn * c->esz);
+    if (c->colsplit || generic) rc |= dalloc(&c->Nh2, c->U * c->nkn * c->esz);
     rc |= dalloc(&c->O, c->nO * c->nk * c->esz);
     rc |= dalloc(&c->mreal, c->U * c->nr * rsz);
     rc |= dalloc(&c->R, c->nO * c->nr * rsz);

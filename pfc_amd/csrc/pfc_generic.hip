@@ -5,13 +5,14 @@
 // Layout: the densities live in real space (mreal, U x Ny x Nx); J, K, Nh
 // and O are half spectra of (Nx/2 + 1) columns of Ny values each (column kx
 // at kx * Ny), the reference's transposed layout.  A step (src/pfc-relax.c:
-// 1133-1247) is six launches:
+// 1133-1247) is three launches:
 //   [t % 100 == 0: Nh = FFT2(m) / (Nx Ny), two launches]
-//   nonlinear term in place (m[0] NaN check)    k_gen_nonlinear
-//   J = FFT_x(m) per row, K = FFT_y(J)           k_gen_row_r2c, k_gen_col_fft
-//   Nh = A Nh + B (K + couplings)               k_gen_update (+ FP32 Hermitian
-//                                                 projection of kx = 0, Nx/2)
-//   J = IFFT_y(Nh), m = C2R_x(J)                 k_gen_col_fft, k_gen_row_c2r
+//   per row and field: the nonlinear term (m[0] NaN check) and its FFT_x
+//     into J                                     k_gen_row_r2c_nl
+//   per column and field: FFT_y(J), Nh2 = A Nh + B (FFT_y(J) + couplings)
+//     (+ FP32 Hermitian projection of kx = 0, Nx/2), J = IFFT_y(Nh2);
+//     Nh and Nh2 then swap                       k_gen_col_step
+//   m = C2R_x(J)                                 k_gen_row_c2r
 // Every transform is one block per row or column with the sequence in LDS,
 // so the real-space and spectral sweeps are separate (more HBM traffic than
 // the fused tuned path, which is what this path trades for generality).
@@ -190,6 +191,36 @@ __global__ __launch_bounds__(kGenThreads) void k_gen_row_r2c(const T* __restrict
     for (int k = threadIdx.x; k <= Nx / 2; k += blockDim.x) d[(size_t)k * Ny + y] = r[k];
 }
 
+// J_f[kx][y] = FFT_x(nonlinear term of field f of row y)[kx] (one block per
+// row y and field f): the nonlinear term of step() (src/pfc-relax.c:
+// 1187-1208) from all fields' densities of the row (nonlinear(), as the
+// tuned row pass computes it), without writing it back; block (0, 0) makes the NaN check of
+// field 0 at (0, 0) (:1175)
+template <typename T, int U>
+__global__ __launch_bounds__(kGenThreads) void k_gen_row_r2c_nl(
+    const T* __restrict__ m, cpx<T>* __restrict__ dst, int Nx, int Ny, size_t rstride,
+    size_t kstride, const DevParams* __restrict__ prm, int* __restrict__ nan_step, int tstep,
+    GenTables<T> gx) {
+    extern __shared__ unsigned char smem_nl[];
+    cpx<T>* a = (cpx<T>*)smem_nl;
+    cpx<T>* b = a + gen_lds_elems(gx.p) / 2;
+    const int y = blockIdx.x, f = blockIdx.y;
+    for (int x = threadIdx.x; x < Nx; x += blockDim.x) {
+        T v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) v[u] = m[u * rstride + (size_t)y * Nx + x];
+        if (f == 0 && y == 0 && x == 0 && v[0] != v[0]) atomicMin(nan_step, tstep);
+        nonlinear<U, T>(prm, v);
+        T w = v[0];
+#pragma unroll
+        for (int u = 1; u < U; u++) w = u == f ? v[u] : w;
+        a[x] = {w, (T)0};
+    }
+    const cpx<T>* r = gen_fft<-1, T>(a, b, gx);
+    cpx<T>* d = dst + f * kstride;
+    for (int k = threadIdx.x; k <= Nx / 2; k += blockDim.x) d[(size_t)k * Ny + y] = r[k];
+}
+
 // m_b[y] = C2R_x(J_b[.][y]): the row's half spectrum extended by Hermitian
 // symmetry, the imaginary parts of kx = 0 and kx = Nx/2 dropped, as a 1D c2r
 // does
@@ -229,21 +260,6 @@ __global__ __launch_bounds__(kGenThreads) void k_gen_col_fft(const cpx<T>* src, 
     for (int j = threadIdx.x; j < Ny; j += blockDim.x) dst[off + j] = cscale(r[j], (T)scale);
 }
 
-// The nonlinear term of step() (src/pfc-relax.c:1187-1208) in place, after
-// the NaN check of field 0 at (0, 0) (:1175)
-template <typename T, int U>
-__global__ __launch_bounds__(256) void k_gen_nonlinear(T* __restrict__ m, size_t nr, const DevParams* __restrict__ prm,
-                                int* __restrict__ nan_step, int tstep) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nr; i += (size_t)gridDim.x * blockDim.x) {
-        T v[U];
-#pragma unroll
-        for (int u = 0; u < U; u++) v[u] = m[u * nr + i];
-        if (i == 0 && v[0] != v[0]) atomicMin(nan_step, tstep);
-        nonlinear<U, T>(prm, v);
-#pragma unroll
-        for (int u = 0; u < U; u++) m[u * nr + i] = v[u];
-    }
-}
 
 // k^2 of spectral index idx (column kx = idx / Ny, row j), rounded as the
 // reference forms it (src/pfc-relax.c:1217-1222)
@@ -254,58 +270,68 @@ __device__ __forceinline__ double gen_k2(size_t idx, int Ny, const DevParams* __
     return kxd * kxd + ky * ky;
 }
 
-// The k-space update of step() (:1214-1242) at every k: Nh <- A Nh + B (K +
-// couplings).  The FP32 form carries k = 0 in FP64 (n0), as the tuned path.
+
+// The column pass of a step, per field (one block per column kx and field
+// f): FFT_y of J_f, the k-space update of field f (src/pfc-relax.c:
+// 1214-1242, kupdate_one: the other fields' old states read from Nsrc, the
+// new state of f written to Ndst), in the FP32 form the Hermitian projection
+// of columns kx = 0 and Nx/2, then IFFT_y back into J_f.  Replaces a
+// forward column transform into K, a separate update sweep and an inverse
+// column transform: no K sweep and no update sweep.  The mixed form's k = 0
+// state is read from n0src and written to n0dst.
 template <typename T, int U>
-__global__ __launch_bounds__(256) void k_gen_update(const cpx<T>* __restrict__ K, cpx<T>* __restrict__ Nh, size_t nk,
-                             int Ny, const DevParams* __restrict__ prm, double* __restrict__ n0) {
-    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < nk;
-         idx += (size_t)gridDim.x * blockDim.x) {
-        const double k2 = gen_k2(idx, Ny, prm);
-        double G = exp(prm->sig_a * k2);
-        G *= G;
-        cpx<T> M[U], N[U];
+__global__ __launch_bounds__(kGenThreads) void k_gen_col_step(
+    cpx<T>* J, const cpx<T>* __restrict__ Nsrc, cpx<T>* __restrict__ Ndst, int Nx, int Ny,
+    size_t nk, const DevParams* __restrict__ prm, const double* __restrict__ n0src,
+    double* __restrict__ n0dst, GenTables<T> gy) {
+    extern __shared__ unsigned char smem_cs[];
+    cpx<T>* a = (cpx<T>*)smem_cs;
+    cpx<T>* b = a + gen_lds_elems(gy.p) / 2;
+    const int kx = blockIdx.x, f = blockIdx.y;
+    const size_t col = (size_t)kx * Ny;
+    cpx<T>* Jf = J + f * nk + col;
+    for (int j = threadIdx.x; j < Ny; j += blockDim.x) a[j] = Jf[j];
+    cpx<T>* r = gen_fft<-1, T>(a, b, gy);
+    cpx<T>* o = (r == a) ? b : a;
+    const bool herm = sizeof(T) == 4 && (kx == 0 || (Nx % 2 == 0 && 2 * kx == Nx));
+    for (int j = threadIdx.x; j < Ny; j += blockDim.x) {
+        const size_t idx = col + j;
+        cpx<T> N[U];
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-            M[u] = K[u * nk + idx];
-            N[u] = Nh[u * nk + idx];
-        }
+        for (int v = 0; v < U; v++) N[v] = Nsrc[v * nk + idx];
+        cpx<T> x;
         if (sizeof(T) == 4 && idx == 0) {
-            cpx<double> M0[U], N0[U];
+            cpx<double> N0[U];
 #pragma unroll
-            for (int u = 0; u < U; u++) {
-                M0[u] = {(double)M[u].x, (double)M[u].y};
-                N0[u] = {n0[2 * u], n0[2 * u + 1]};
-            }
-            kupdate<U, double>(prm, 0.0, 1.0, M0, N0);
-#pragma unroll
-            for (int u = 0; u < U; u++) {
-                n0[2 * u] = N0[u].x;
-                n0[2 * u + 1] = N0[u].y;
-                N[u] = {(T)N0[u].x, (T)N0[u].y};
-            }
+            for (int v = 0; v < U; v++) N0[v] = {n0src[2 * v], n0src[2 * v + 1]};
+            const cpx<double> x0 = kupdate_one<U, double>(
+                prm, f, 0.0, 1.0, cpx<double>{(double)r[j].x, (double)r[j].y}, N0);
+            n0dst[2 * f] = x0.x;
+            n0dst[2 * f + 1] = x0.y;
+            x = {(T)x0.x, (T)x0.y};
         } else {
-            kupdate<U, T>(prm, (T)k2, (T)G, M, N);
+            const double k2 = gen_k2(idx, Ny, prm);
+            double G = exp(prm->sig_a * k2);
+            G *= G;
+            x = kupdate_one<U, T>(prm, f, (T)k2, (T)G, r[j], N);
         }
-#pragma unroll
-        for (int u = 0; u < U; u++) Nh[u * nk + idx] = N[u];
+        r[j] = x;
+        if (!herm) Ndst[f * nk + idx] = x;
     }
+    if (herm) {
+        __syncthreads();
+        for (int j = threadIdx.x; j < Ny; j += blockDim.x)
+            o[j] = cscale(cadd(r[j], cconj(r[j == 0 ? 0 : Ny - j])), (T)0.5);
+        __syncthreads();
+        for (int j = threadIdx.x; j < Ny; j += blockDim.x) {
+            r[j] = o[j];
+            Ndst[f * nk + col + j] = o[j];
+        }
+    }
+    const cpx<T>* q = gen_fft<+1, T>(r, o, gy);
+    for (int j = threadIdx.x; j < Ny; j += blockDim.x) Jf[j] = q[j];
 }
 
-// FP32 form: columns kx = 0 and kx = Nx/2 (Nx even) of the spectral state
-// replaced by their Hermitian part, N(ky) <- (N(ky) + conj N(-ky)) / 2
-// (one block per column and field; blockIdx.x 0: kx = 0, 1: kx = Nx/2)
-template <typename T>
-__global__ __launch_bounds__(kGenThreads) void k_gen_hermitian(cpx<T>* Nh, int Nx, int Ny, size_t nk) {
-    extern __shared__ unsigned char smem_h[];
-    cpx<T>* a = (cpx<T>*)smem_h;
-    const int kx = blockIdx.x == 0 ? 0 : Nx / 2;
-    cpx<T>* col = Nh + blockIdx.y * nk + (size_t)kx * Ny;
-    for (int j = threadIdx.x; j < Ny; j += blockDim.x) a[j] = col[j];
-    __syncthreads();
-    for (int j = threadIdx.x; j < Ny; j += blockDim.x)
-        col[j] = cscale(cadd(a[j], cconj(a[j == 0 ? 0 : Ny - j])), (T)0.5);
-}
 
 // Evaluate's k-space outputs (src/pfc-relax.c:1048-1077) from the refreshed
 // state: O = [P_u N_u (u < U), G N_u (u < U), (c2_uv - k^2)^2 N_v (u < v)]
@@ -370,16 +396,23 @@ struct Gen {
     }
     template <int U>
     static void step_u(pfc_ctx* c, int t) {
-        hipLaunchKernelGGL((k_gen_nonlinear<T, U>), grid1(c->nr), dim3(256), 0, c->stream, (T*)c->mreal,
-                           c->nr, c->dprm, c->nan_flag, t);
-        row_r2c(c, c->mreal, c->J, c->U);
-        col_fft(c, c->J, c->K, c->U, -1, 1.0);
-        hipLaunchKernelGGL((k_gen_update<T, U>), grid1(c->nkn), dim3(256), 0, c->stream, (const C*)c->K,
-                           (C*)c->Nh, c->nkn, c->Ny, c->dprm, c->n0);
+        const GenDim& gxd = ((GenCtx*)c->gen)->x;
+        hipLaunchKernelGGL((k_gen_row_r2c_nl<T, U>), dim3(c->Ny, U), dim3(kGenThreads), lds(gxd),
+                           c->stream, (const T*)c->mreal, (C*)c->J, c->Nx, c->Ny, c->nr, c->nkn,
+                           c->dprm, c->nan_flag, t, tables<T>(gxd));
+        // the column pass per field into the second state buffer, which then
+        // becomes the state (and the mixed form's new k = 0 state is copied
+        // over the old one)
+        const GenDim& g = ((GenCtx*)c->gen)->y;
+        hipLaunchKernelGGL((k_gen_col_step<T, U>), dim3(c->Nx / 2 + 1, U), dim3(kGenThreads), lds(g),
+                           c->stream, (C*)c->J, (const C*)c->Nh, (C*)c->Nh2, c->Nx, c->Ny, c->nkn,
+                           c->dprm, (const double*)c->n0, c->n0 + 2 * U, tables<T>(g));
         if (sizeof(T) == 4)
-            hipLaunchKernelGGL(k_gen_hermitian<T>, dim3(c->Nx % 2 == 0 ? 2 : 1, c->U), dim3(kGenThreads),
-                               (size_t)c->Ny * sizeof(C), c->stream, (C*)c->Nh, c->Nx, c->Ny, c->nkn);
-        col_fft(c, c->Nh, c->J, c->U, +1, 1.0);
+            (void)hipMemcpyAsync(c->n0, c->n0 + 2 * U, 2 * U * sizeof(double), hipMemcpyDeviceToDevice,
+                                 c->stream);
+        void* s = c->Nh;
+        c->Nh = c->Nh2;
+        c->Nh2 = s;
         row_c2r(c, c->J, c->mreal, c->U);
     }
     static void step(pfc_ctx* c, int t) {
@@ -463,8 +496,16 @@ void set_lds_limits() {
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)kGenMaxLds);
     (void)hipFuncSetAttribute((const void*)k_gen_col_fft<T, +1>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)kGenMaxLds);
-    (void)hipFuncSetAttribute((const void*)k_gen_hermitian<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)kGenMaxLds);
+    const void* cs[] = {(const void*)k_gen_col_step<T, 1>, (const void*)k_gen_col_step<T, 2>,
+                        (const void*)k_gen_col_step<T, 3>, (const void*)k_gen_col_step<T, 4>,
+                        (const void*)k_gen_col_step<T, 5>, (const void*)k_gen_col_step<T, 6>,
+                        (const void*)k_gen_col_step<T, 7>, (const void*)k_gen_col_step<T, 8>,
+                        (const void*)k_gen_row_r2c_nl<T, 1>, (const void*)k_gen_row_r2c_nl<T, 2>,
+                        (const void*)k_gen_row_r2c_nl<T, 3>, (const void*)k_gen_row_r2c_nl<T, 4>,
+                        (const void*)k_gen_row_r2c_nl<T, 5>, (const void*)k_gen_row_r2c_nl<T, 6>,
+                        (const void*)k_gen_row_r2c_nl<T, 7>, (const void*)k_gen_row_r2c_nl<T, 8>};
+    for (const void* k : cs)
+        (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kGenMaxLds);
 }
 
 }  // namespace
