@@ -1767,7 +1767,7 @@ __global__ __launch_bounds__(BS) void k_energy_local(const T* __restrict__ n, si
 // groups (XCD-contiguous, persist_row_group); each block writes one set of
 // partial sums.
 template <typename T, int NX, int TPB, int TR, int U>
-__global__ __launch_bounds__(TPB* TR) void k_row_c2r_eloc(
+__global__ __launch_bounds__(TPB* TR) PFC_WAVES((row_waves<T, NX>())) void k_row_c2r_eloc(
     const cpx<T>* __restrict__ I, cpx<T>* __restrict__ K, size_t fstride, int sI, int sK,
     const cpx<T>* __restrict__ twx_, const DevParams* __restrict__ p,
     double* __restrict__ partials, int ngroups) {
