@@ -31,9 +31,10 @@ def state(Nx, Ny, U, seed):
 
 
 # (Nx, Ny, U): tuned rows/columns of several tilings, 4096-point columns
-# (per-field column pass on the plain steps), 8192-point rows, and a generic
-# shape
-SHAPES = [(64, 64, 3), (128, 64, 1), (64, 4096, 3), (8192, 64, 3), (256, 256, 1), (100, 90, 3)]
+# (per-field column pass on the plain steps), 4096- and 8192-point rows (the
+# persistent FP64 row pass), and a generic shape
+SHAPES = [(64, 64, 3), (128, 64, 1), (64, 4096, 3), (4096, 64, 3), (8192, 64, 3), (256, 256, 1),
+          (100, 90, 3)]
 
 
 @pytest.mark.parametrize("precision", [FP64, MIXED])
@@ -59,6 +60,26 @@ def test_matches_evaluate_then_step(gpu, Nx, Ny, U, precision, t0, n):
         a.get_energy()                          # as after pfc_evaluate_sums
     a.close()
     b.close()
+
+
+def test_headline_grid(gpu):
+    """4096^2 with three fields (the benchmark's grid and kernels, FP64): a
+    print at step 100 fused into the refresh step against pfc_evaluate_sums
+    then pfc_step, to 1e-13 (the same sums over other block partitions)."""
+    N = 4096
+    p = hetero_params(dt=0.25, c=1.0, sigma=2.0 / 7.2552)
+    f = hetero_patchwork(N, N, 0.7, 0.7, domain=256, seed=2024, noise=0.01)
+    with Engine(p, N, N, 0.7, 0.7, FP64) as a, Engine(p, N, N, 0.7, 0.7, FP64) as b:
+        for g in (a, b):
+            g.set_fields(f)
+            g.step(91, 9)
+        fa, na = a.step_evaluate(100, 3)
+        fb, nb = b.evaluate(density=False)
+        b.step(100, 3)
+        assert abs(fa - fb) <= 1e-13 * abs(fb), (fa, fb)
+        assert np.allclose(na, nb, rtol=1e-13, atol=1e-15)
+        assert rel_l2(a.get_fields(), b.get_fields()) < 1e-13
+        assert a.nan_step() == -1 and b.nan_step() == -1
 
 
 @pytest.mark.parametrize("Nx,Ny", [(128, 64), (64, 4096)])
