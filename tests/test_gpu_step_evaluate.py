@@ -30,11 +30,11 @@ def state(Nx, Ny, U, seed):
     return graphene_params(), -0.2 + 0.05 * rng.standard_normal((1, Ny, Nx))
 
 
-# (Nx, Ny, U): tuned rows/columns of several tilings, 4096-point columns
+# (Nx, Ny, U): tuned rows/columns of several tilings, 4096- and 8192-point columns
 # (per-field column pass on the plain steps), 4096- and 8192-point rows (the
 # persistent FP64 row pass), and a generic shape
-SHAPES = [(64, 64, 3), (128, 64, 1), (64, 4096, 3), (4096, 64, 3), (8192, 64, 3), (256, 256, 1),
-          (100, 90, 3)]
+SHAPES = [(64, 64, 3), (128, 64, 1), (64, 4096, 3), (64, 8192, 3), (4096, 64, 3), (8192, 64, 3),
+          (256, 256, 1), (100, 90, 3)]
 
 
 @pytest.mark.parametrize("precision", [FP64, MIXED])
