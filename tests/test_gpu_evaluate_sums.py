@@ -58,7 +58,8 @@ def state(kind, Nx, Ny):
 # tuned kernels (2^k and 3 * 2^k sides, U <= 4), then the generic path
 # (other sides, odd Nx and odd Ny, U = 5)
 CASES = [("graphene", 128, 64), ("two fields", 128, 64), ("hetero", 256, 256),
-         ("hetero nu2 alias", 256, 128), ("hetero", 384, 192), ("graphene", 100, 90),
+         ("hetero nu2 alias", 256, 128), ("hetero", 384, 192), ("hetero", 4096, 64),
+         ("hetero", 64, 4096), ("hetero", 8192, 64), ("graphene", 100, 90),
          ("hetero", 97, 64), ("hetero", 64, 45), ("five fields", 64, 64)]
 
 
