@@ -22,7 +22,7 @@ def rel_l2(a, b):
 
 
 @pytest.mark.parametrize("precision", [FP64, MIXED])
-@pytest.mark.parametrize("Nx,Ny", [(128, 128), (64, 4096)])
+@pytest.mark.parametrize("Nx,Ny", [(128, 128), (64, 4096), (4096, 64)])
 def test_skip_matches_forced_refresh(gpu, Nx, Ny, precision):
     p = hetero_params()
     f = hetero_patchwork(Nx, Ny, 0.7, 0.7, domain=32, seed=9)
