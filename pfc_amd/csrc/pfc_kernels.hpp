@@ -1355,6 +1355,16 @@ __global__ __launch_bounds__(TPB* TC) PFC_WAVES((colf_waves<T, NY>())) void k_co
     const unsigned vi = lay_col_off(sI, H, jb.mode ? 0 : kx, t, E);
     const unsigned istep = lay_col_step(sI, H, TPB, E);
 
+    // FP64 columns of 4096: the waves issue their loads (table, J, each
+    // batch of states) at a raised priority, so that on a CU shared by two
+    // blocks the memory requests of one go out before the other block's
+    // transform work (4096^2, three fields: 394 -> 390 us).  Columns of 8192
+    // lost with it (FP64 1.746 -> 1.762 ms, FP32 0.832 -> 0.857 ms).
+#ifndef PFC_COLF_PRIO
+#define PFC_COLF_PRIO 2
+#endif
+    constexpr int PRIO = (sizeof(T) == 8 && NY == 4096) ? PFC_COLF_PRIO : 0;
+    if (PRIO) __builtin_amdgcn_s_setprio(PRIO);
     // the table gather first, then the field's J (the LDS write of the
     // table waits for the gather only)
     cpx<T> tv[NI];
@@ -1372,6 +1382,7 @@ __global__ __launch_bounds__(TPB* TC) PFC_WAVES((colf_waves<T, NY>())) void k_co
         for (int q = 0; q < P; q++) M[q] = col_in<T>(src, vr, q * rstep, jb.mode);
     }
     __builtin_amdgcn_sched_barrier(0);
+    if (PRIO) __builtin_amdgcn_s_setprio(0);
 #pragma unroll
     for (int i = 0; i < NI; i++) {
         const int e = tid + i * NT;
@@ -1390,6 +1401,7 @@ __global__ __launch_bounds__(TPB* TC) PFC_WAVES((colf_waves<T, NY>())) void k_co
         for (int q0 = 0; q0 < P; q0 += QB) {
             cpx<T> Nb[QB][U];
             T gyb[QB];
+            if (PRIO) __builtin_amdgcn_s_setprio(PRIO);
 #pragma unroll
             for (int i = 0; i < QB; i++) {
 #pragma unroll
@@ -1397,6 +1409,7 @@ __global__ __launch_bounds__(TPB* TC) PFC_WAVES((colf_waves<T, NY>())) void k_co
                     Nb[i][v] = gload<T>(gbuf(Nsrc + v * nstride), vn, (q0 + i) * TPB * E);
                 gyb[i] = gload_real<T>(ktab, (unsigned)(ktab_gy(H) + t + (q0 + i) * TPB) * sizeof(T));
             }
+            if (PRIO) __builtin_amdgcn_s_setprio(0);
 #pragma unroll
             for (int i = 0; i < QB; i++) {
                 const int q = q0 + i;
